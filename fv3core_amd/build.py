@@ -1,0 +1,92 @@
+"""Builds ``libfv3core_amd.so`` (the gfx950 kernels behind the C ABI) in-tree.
+
+Every ``csrc/*.hip`` file is compiled by hipcc to an object for gfx950 and
+the objects are linked into one shared library next to this file.  The
+flags:
+
+* ``-ffp-contract=off`` keeps every multiply and add separately rounded, so
+  the FP64 kernels evaluate the reference's expressions in the same order and
+  rounding as numpy does;
+* ``-Xclang -target-feature -Xclang -packed-fp32-ops`` turns off packed FP32
+  instructions (the library never relies on them).
+
+Run ``python -m fv3core_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import concurrent.futures
+import os
+import pathlib
+import shutil
+import subprocess
+import sys
+
+HERE = pathlib.Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+LIB_NAME = "libfv3core_amd.so"
+OFFLOAD_ARCH = "gfx950"
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found (ROCm is required to build fv3core_amd)")
+
+
+COMMON_FLAGS = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    f"--offload-arch={OFFLOAD_ARCH}",
+    "-ffp-contract=off",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _compile(src: pathlib.Path, obj: pathlib.Path) -> None:
+    cmd = [hipcc()] + COMMON_FLAGS + ["-c", str(src), "-o", str(obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stdout}\n{res.stderr}")
+    # hipcc's host pass does not know the packed-fp32 target feature and says
+    # so; every other diagnostic is shown.
+    for line in (res.stderr or "").splitlines():
+        if "packed-fp32-ops" in line or not line.strip():
+            continue
+        print(f"[{src.name}] {line}", file=sys.stderr)
+
+
+def build(verbose: bool = True) -> pathlib.Path:
+    """Compile every HIP source and link the shared library; returns its path."""
+    objdir = HERE / "_build"
+    objdir.mkdir(exist_ok=True)
+    sources = sorted(CSRC.glob("*.hip"))
+    if not sources:
+        raise RuntimeError(f"no HIP sources under {CSRC}")
+    objs = [objdir / (s.stem + ".o") for s in sources]
+    workers = min(len(sources), max(1, (os.cpu_count() or 2) // 2), 8)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        futs = [ex.submit(_compile, s, o) for s, o in zip(sources, objs)]
+        for f in futs:
+            f.result()
+    out = HERE / LIB_NAME
+    tmp = HERE / (LIB_NAME + ".tmp")
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={OFFLOAD_ARCH}"] + [
+        str(o) for o in objs
+    ] + ["-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out} from {len(sources)} sources")
+    return out
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,135 @@
+/*
+ * fv3core_amd: C ABI of the MI355X (gfx950) FV3 dynamical-core kernels.
+ *
+ * Every entry point
+ *   - takes plain device pointers to caller-owned storages, never frees them;
+ *   - only enqueues work on the given HIP stream (a hipStream_t passed as
+ *     void*, NULL = default stream) and never synchronises, so it can be
+ *     captured in a HIP graph;
+ *   - returns 0 on success or a negative FV3_E* code; fv3_last_error()
+ *     returns a thread-local message for the last failure on this thread.
+ *
+ * Storage layout (fv3_domain): a field of the reference's storage shape
+ * (ni, nj, nk) is addressed as base[t*st + k*sk + j*sj + i] (i fastest, unit
+ * stride), t being the tile/member index of a batch.  IJ metric fields use
+ * base[t*st2 + j*sj2 + i].  Indices are the reference's storage indices:
+ * the compute domain is [isc, iec] x [jsc, jec] (isc = jsc = n_halo).
+ *
+ * The reference exposes these operations as Python classes; the comment on
+ * each entry point names the class method it replaces.
+ */
+#ifndef FV3CORE_AMD_H
+#define FV3CORE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FV3_OK 0
+#define FV3_E_ARG (-1)          /* invalid argument (null pointer, bad size) */
+#define FV3_E_UNSUPPORTED (-2)  /* option the reference also rejects */
+#define FV3_E_HIP (-3)          /* a HIP runtime call failed */
+
+/* Anchors for a tile edge this domain does not own: far enough outside any
+ * storage that no region test can match (the reference shifts the anchor by
+ * the int16 maximum for the same purpose, fv3core/utils/stencil.py:457-495). */
+#define FV3_NO_EDGE_LO (-(1 << 28))
+#define FV3_NO_EDGE_HI (1 << 28)
+
+typedef struct fv3_domain {
+  int32_t isc, iec, jsc, jec;           /* compute domain (storage indices) */
+  int32_t i_start, i_end, j_start, j_end; /* tile-edge anchors or sentinels */
+  int32_t ni, nj;                       /* storage extents in i and j */
+  int32_t nk;                           /* number of levels processed */
+  int32_t ntiles;                       /* batch extent (tiles x members) */
+  int64_t sj, sk, st;                   /* 3-D strides in elements */
+  int64_t sj2, st2;                     /* IJ strides in elements */
+} fv3_domain;
+
+/* Library version string. */
+const char* fv3_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fv3_last_error(void);
+
+/* Number of gfx950 devices visible (0 on a machine without a GPU). */
+int32_t fv3_device_count(void);
+
+/*
+ * CopyCornersXY.__call__ (fv3core/utils/corners.py:64-117, stencil
+ * copy_corners_xy_stencil_defn corners.py:425-540):
+ * q_x <- q with x-corner copies (in place when q_x == q), q_y <- q with
+ * y-corner copies.  Processes the full storage extent in i and j and nk levels.
+ */
+int32_t fv3_copy_corners_xy_f64(const fv3_domain* d, const double* q, double* q_x,
+                                double* q_y, void* stream);
+int32_t fv3_copy_corners_xy_f32(const fv3_domain* d, const float* q, float* q_x,
+                                float* q_y, void* stream);
+
+/*
+ * FiniteVolumeTransport.__call__ without damping (fv3core/stencils/fvtp2d.py:249-360),
+ * i.e. y_ppm inner -> q_i -> x_ppm outer, x_ppm inner -> q_j -> y_ppm outer,
+ * final_fluxes, fused in one kernel.  hord in {5, 6, 7, 8}.
+ * x_mass_flux / y_mass_flux may be NULL (then the area fluxes are the unit
+ * fluxes).  Writes q_x_flux on [isc, iec+1] x [jsc, jec] and q_y_flux on
+ * [isc, iec] x [jsc, jec+1] for nk levels.
+ */
+int32_t fv3_fvtp2d_f64(const fv3_domain* d, int32_t hord, const double* q_x,
+                       const double* q_y, const double* crx, const double* cry,
+                       const double* x_area_flux, const double* y_area_flux,
+                       const double* x_mass_flux, const double* y_mass_flux,
+                       const double* area, const double* dxa, const double* dya,
+                       double* q_x_flux, double* q_y_flux, void* stream);
+int32_t fv3_fvtp2d_f32(const fv3_domain* d, int32_t hord, const float* q_x,
+                       const float* q_y, const float* crx, const float* cry,
+                       const float* x_area_flux, const float* y_area_flux,
+                       const float* x_mass_flux, const float* y_mass_flux,
+                       const float* area, const float* dxa, const float* dya,
+                       float* q_x_flux, float* q_y_flux, void* stream);
+
+/*
+ * XPiecewiseParabolic.__call__ / YPiecewiseParabolic.__call__
+ * (fv3core/stencils/xppm.py:283-348, yppm.py): mean of q advected through
+ * the x (direction 0) or y (direction 1) interfaces of the box
+ * [oi, oi+ei) x [oj, oj+ej), for nk levels; ord = iord/jord in {5,6,7,8};
+ * dxa is dxa (x) or dya (y).
+ */
+int32_t fv3_ppm_flux_f64(const fv3_domain* d, int32_t direction, int32_t ord, int32_t oi,
+                         int32_t oj, int32_t ei, int32_t ej, const double* q, const double* c,
+                         const double* dxa, double* out, void* stream);
+int32_t fv3_ppm_flux_f32(const fv3_domain* d, int32_t direction, int32_t ord, int32_t oi,
+                         int32_t oj, int32_t ei, int32_t ej, const float* q, const float* c,
+                         const float* dxa, float* out, void* stream);
+
+/* DelnFlux modes */
+#define FV3_DELN_ADD 0       /* fx += fx2, fy += fy2 (DelnFlux, mass is None) */
+#define FV3_DELN_MASS 1      /* fx += 0.5*damp*(mass[i-1]+mass)*fx2 (DelnFlux with mass) */
+#define FV3_DELN_STORE 2     /* fx = fx2, fy = fy2 (DelnFluxNoSG, damp*q preamble) */
+#define FV3_DELN_STORE_MASS 3 /* fx = fx2, fy = fy2 (DelnFluxNoSG with mass: d2 = q) */
+
+/*
+ * DelnFlux.__call__ (fv3core/stencils/delnflux.py:955-998) and
+ * DelnFluxNoSG.__call__ (delnflux.py:1156-1207), fused: del-(2*nord+2)
+ * diffusive fluxes with the in-between corner copies, for levels [0, nk).
+ * nord4 holds nord for the level bands k = 0, 1, 2 and k >= 3 (each in
+ * [0, 3], max <= 2: nord = 3 would need a 4-cell halo); damp is a device array of nk per-level coefficients
+ * ((damp_c*da_min)^(nord+1) for DelnFlux).  mass may be NULL unless the mode
+ * needs it.  fx/fy are updated on [isc, iec+1] x [jsc, jec] and
+ * [isc, iec] x [jsc, jec+1].
+ */
+int32_t fv3_delnflux_f64(const fv3_domain* d, int32_t mode, const int32_t nord4[4],
+                         const double* q, const double* mass, const double* damp,
+                         const double* del6_u, const double* del6_v,
+                         const double* rarea, double* fx, double* fy, void* stream);
+int32_t fv3_delnflux_f32(const fv3_domain* d, int32_t mode, const int32_t nord4[4],
+                         const float* q, const float* mass, const float* damp,
+                         const float* del6_u, const float* del6_v,
+                         const float* rarea, float* fx, float* fy, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FV3CORE_AMD_H */
