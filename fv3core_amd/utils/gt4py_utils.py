@@ -45,20 +45,29 @@ def to_torch_dtype(dtype) -> torch.dtype:
     return _DTYPES[np.dtype(dtype).type]
 
 
-def empty(shape, dtype=np.float64, device=None) -> torch.Tensor:
-    """Uninitialised storage with i-fastest memory order."""
+def empty(shape, dtype=np.float64, device=None, batch_dims=None) -> torch.Tensor:
+    """Uninitialised storage with i-fastest memory order.
+
+    The last ``len(shape) - batch_dims`` axes are the field axes (I, J[, K]),
+    stored with I fastest; leading batch axes (tiles, members) are slowest.
+    By default a 4-D shape has one batch axis and 1- to 3-D shapes none.
+    """
     device = default_device() if device is None else device
     tdt = to_torch_dtype(dtype)
     shape = tuple(int(s) for s in shape)
-    if len(shape) <= 1:
+    if batch_dims is None:
+        batch_dims = max(0, len(shape) - 3)
+    field = shape[batch_dims:]
+    if len(field) <= 1:
         return torch.empty(shape, dtype=tdt, device=device)
-    # allocate reversed and permute: the first axis becomes the fastest
-    rev = torch.empty(shape[::-1], dtype=tdt, device=device)
-    return rev.permute(*range(len(shape) - 1, -1, -1))
+    # allocate the field axes reversed and permute them back: I becomes fastest
+    rev = torch.empty(shape[:batch_dims] + field[::-1], dtype=tdt, device=device)
+    perm = list(range(batch_dims)) + [batch_dims + len(field) - 1 - a for a in range(len(field))]
+    return rev.permute(*perm)
 
 
-def zeros(shape, dtype=np.float64, device=None) -> torch.Tensor:
-    t = empty(shape, dtype, device)
+def zeros(shape, dtype=np.float64, device=None, batch_dims=None) -> torch.Tensor:
+    t = empty(shape, dtype, device, batch_dims)
     t.zero_()
     return t
 

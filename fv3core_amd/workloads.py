@@ -115,3 +115,112 @@ def uniform_metric(n: int, n_halo: int = 3, dx: float = 1.0e5):
         del6_v=ones * (dx / dx),
         da_min=float(dx * dx),
     )
+
+
+def hybrid_coefficients(km: int = 79):
+    """(ak, bk, ptop) of the reference's 79-layer hybrid grid (fv3core/grid/eta.py:38-217)."""
+    if km != 79:
+        raise NotImplementedError("Only a 79 vertical level grid is tabulated")
+    import pathlib
+
+    d = np.load(pathlib.Path(__file__).with_name("_eta79.npz"))
+    ak, bk = d["ak"].copy(), d["bk"].copy()
+    return ak, bk, float(ak[0])
+
+
+def _smooth_ij(rng, shape2, amp):
+    ni, nj = shape2
+    x = np.arange(ni)[:, None] / max(ni - 1, 1)
+    y = np.arange(nj)[None, :] / max(nj - 1, 1)
+    f = np.zeros(shape2)
+    for m in (1, 2):
+        a, p1, p2 = rng.uniform(0.3, 1.0), rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi)
+        f += a * np.sin(2 * np.pi * m * x + p1) * np.cos(2 * np.pi * m * y + p2)
+    return amp * f / np.abs(f).max()
+
+
+def remap_inputs(n: int = 48, nz: int = 79, workload: int = 4, n_halo: int = 3):
+    """Eulerian and perturbed Lagrangian interfaces plus fields to remap.
+
+    pe2 = ak + bk*ps with ps = 1e5 Pa + a smooth +-500 Pa field; pe1 = pe2
+    plus a smooth interior displacement (zero at top and bottom, at most 30%
+    of the local layer thickness), so both grids span the same column.
+    Returns dict(pe1, pe2, pt (K-like scalar), q (tracer in [0, 1e-2] with
+    zeros), w (signed), qs (IJ bottom value), ptop).
+    """
+    ak, bk, ptop = hybrid_coefficients(nz)
+    shape = storage_shape(n, nz, n_halo)
+    ni, nj, nk = shape
+    base = 1000 * workload
+    rng = np.random.default_rng(base + 5)
+    ps = 1.0e5 + _smooth_ij(rng, (ni, nj), 500.0)
+    pe2 = ak[None, None, :] + bk[None, None, :] * ps[:, :, None]
+    dp2 = np.diff(pe2, axis=2)
+    rng2 = np.random.default_rng(base + 6)
+    disp = rng2.uniform(-0.3, 0.3, size=(ni, nj, nk))
+    disp[:, :, 0] = 0.0
+    disp[:, :, -1] = 0.0
+    lim = np.minimum(np.concatenate([dp2[:, :, :1], dp2], axis=2),
+                     np.concatenate([dp2, dp2[:, :, -1:]], axis=2))
+    pe1 = pe2 + disp * lim
+    rng3 = np.random.default_rng(base + 7)
+    kk = np.arange(nz)[None, None, :]
+    pt = 220.0 + 80.0 * (kk / nz) ** 2 + rng3.normal(0, 1.0, size=(ni, nj, nz))
+    q = np.clip(1e-2 * (kk / nz) ** 3 + rng3.normal(0, 1e-3, size=(ni, nj, nz)), 0.0, None)
+    w = rng3.normal(0.0, 1.0, size=(ni, nj, nz))
+    pad = lambda a: np.concatenate([a, a[:, :, -1:]], axis=2)  # noqa: E731
+    return dict(pe1=pe1, pe2=pe2, pt=pad(pt), q=pad(q), w=pad(w),
+                qs=rng3.normal(0.0, 0.5, size=(ni, nj)), ptop=ptop, ak=ak, bk=bk)
+
+
+def riemann_inputs(n: int = 24, nz: int = 79, workload: int = 2, n_halo: int = 3,
+                   dt: float = 37.5):
+    """A hydrostatically consistent column state plus perturbations.
+
+    delp from the hybrid grid with ps ~ 1e5 Pa; cappa = Rd/cp with a small
+    moisture-like variation; T ~ 210-290 K and pt = T / p**cappa; delz from the discrete hydrostatic
+    relation the solver uses (so the unperturbed state is at rest), then pt
+    perturbed by 1 % and w by N(0, 0.5 m/s).
+    """
+    RD, CP, G = 287.05, 1004.6, 9.80665
+    ak, bk, ptop = hybrid_coefficients(nz)
+    shape = storage_shape(n, nz, n_halo)
+    ni, nj, nk = shape
+    base = 1000 * workload
+    rng = np.random.default_rng(base + 11)
+    ps = 1.0e5 + _smooth_ij(rng, (ni, nj), 800.0)
+    pe = ak[None, None, :] + bk[None, None, :] * ps[:, :, None]
+    delp = np.zeros(shape)
+    delp[:, :, :nz] = np.diff(pe, axis=2)
+    delp[:, :, nz] = delp[:, :, nz - 1]
+    q_con = np.zeros(shape)
+    q_con[:, :, :nz] = rng.uniform(0.0, 2e-3, size=(ni, nj, nz))
+    cappa = RD / CP * (1.0 + rng.uniform(-0.01, 0.01, size=shape))
+    kk = np.arange(nk)[None, None, :]
+    temp = 210.0 + 80.0 * (kk / nz) + rng.normal(0, 1.0, size=shape)
+    # hydrostatic dz for the solver's equation of state
+    peg = np.zeros(shape)
+    peg[:, :, 0] = ptop
+    for k in range(1, nk):
+        peg[:, :, k] = peg[:, :, k - 1] + delp[:, :, k - 1] * (1.0 - q_con[:, :, k - 1])
+    pm = np.zeros(shape)
+    pm[:, :, :nz] = (peg[:, :, 1:] - peg[:, :, :nz]) / np.log(peg[:, :, 1:] / peg[:, :, :nz])
+    pm[:, :, nz] = pm[:, :, nz - 1]
+    # the acoustic solver's pt is T / p**cappa (p in Pa)
+    pt = temp * np.exp(-cappa * np.log(pm))
+    dm = delp / G
+    delz = -dm * RD * pt * np.exp((cappa - 1.0) * np.log(pm))
+    delz[:, :, nz] = 0.0
+    zs = np.zeros((ni, nj))
+    zh = np.zeros(shape)
+    zh[:, :, nz] = zs
+    for k in range(nz - 1, -1, -1):
+        zh[:, :, k] = zh[:, :, k + 1] - delz[:, :, k]
+    pt = pt * (1.0 + rng.uniform(-0.01, 0.01, size=shape))
+    w = rng.normal(0.0, 0.5, size=shape)
+    # the C-grid solver reads gz as heights (its dz is gz[k+1] - gz[k]) and
+    # returns geopotential (riem_solver_c.py:36, 57-64)
+    gz = zh.copy()
+    return dict(ak=ak, bk=bk, ptop=ptop, delp=delp, q_con=q_con, cappa=cappa, pt=pt,
+                delz=delz, zh=zh, zs=zs, hs=zs * G, w=w, ws=np.zeros((ni, nj)), gz=gz,
+                pe=pe.copy() if pe.shape[2] == nk else pe, dt=dt)

@@ -128,6 +128,50 @@ int32_t fv3_delnflux_f32(const fv3_domain* d, int32_t mode, const int32_t nord4[
                          const float* del6_u, const float* del6_v,
                          const float* rarea, float* fx, float* fy, void* stream);
 
+/*
+ * MapSingle.__call__ (fv3core/stencils/map_single.py:130-173) with its
+ * RemapProfile (remap_profile.py:576-638): conservative remap of the layer
+ * means q1 from interfaces pe1 to interfaces pe2 on the box
+ * [i1, i2] x [j1, j2], km = d->nk layers (pe1/pe2 hold km + 1 interfaces).
+ * kord <= 10 (its magnitude selects the limiter), mode = iv in {-2,...,2};
+ * qs (IJ, bottom value for iv = -2) may be NULL (then 0); qmin is the
+ * kord-9 floor.  work: 7 slabs, each of work_slab elements laid out like the
+ * fields (work_slab >= (ntiles-1)*st + (km+1)*sk).
+ */
+int32_t fv3_map_single_f64(const fv3_domain* d, int32_t kord, int32_t mode, int32_t i1,
+                           int32_t i2, int32_t j1, int32_t j2, double* q1, const double* pe1,
+                           const double* pe2, const double* qs, double qmin, double* work,
+                           int64_t work_slab, void* stream);
+int32_t fv3_map_single_f32(const fv3_domain* d, int32_t kord, int32_t mode, int32_t i1,
+                           int32_t i2, int32_t j1, int32_t j2, float* q1, const float* pe1,
+                           const float* pe2, const float* qs, double qmin, float* work,
+                           int64_t work_slab, void* stream);
+
+/*
+ * RiemannSolverC.__call__ (fv3core/stencils/riem_solver_c.py:119-165):
+ * precompute + sim1_solver + finalize on [isc-1, iec+1] x [jsc-1, jec+1],
+ * d->nk = nz + 1 levels, FP64.  Outputs gz and pef; w3 is read only.
+ * work: 10 slabs of work_slab elements (>= (ntiles-1)*st + nk*sk).
+ */
+int32_t fv3_riem_solver_c_f64(const fv3_domain* d, double dt2, double ptop, double p_fac,
+                              const double* cappa, const double* hs, const double* ws,
+                              const double* ptc, const double* q_con, const double* delpc,
+                              double* gz, double* pef, const double* w3, double* work,
+                              int64_t work_slab, void* stream);
+
+/*
+ * RiemannSolver3.__call__ (fv3core/stencils/riem_solver3.py:145-242) on the
+ * compute domain, d->nk = nz + 1 levels, FP64 (a_imp > 0.999 path).
+ * In/out: delz, zh, pe, ppe, pk3, pk, peln, w.  work: 8 slabs.
+ */
+int32_t fv3_riem_solver3_f64(const fv3_domain* d, int32_t last_call, double dt, double ptop,
+                             double p_fac, int32_t use_logp, double beta, const double* cappa,
+                             const double* zs, const double* wsd, double* delz,
+                             const double* q_con, const double* delp, const double* pt,
+                             double* zh, double* pe, double* ppe, double* pk3, double* pk,
+                             double* peln, double* w, double* work, int64_t work_slab,
+                             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import torch
 
-from fv3core_amd.stencils.delnflux import DelnFlux, DelnFluxNoSG, calc_damp
+from fv3core_amd.stencils.delnflux import DelnFluxNoSG
 from fv3core_amd.stencils.fvtp2d import (
     CopiedCorners,
     FiniteVolumeTransport,
@@ -262,4 +262,3 @@ def test_unsupported_and_bad_arguments_raise():
     with pytest.raises(ValueError):
         bad = torch.zeros(3, 3, 3, dtype=torch.float64, device="cuda")
         fvt(CopiedCorners(q, q, q), q, q, q, q, bad, bad)
-    del calc_damp, DelnFlux
