@@ -172,6 +172,42 @@ int32_t fv3_riem_solver3_f64(const fv3_domain* d, int32_t last_call, double dt, 
                              double* peln, double* w, double* work, int64_t work_slab,
                              void* stream);
 
+/*
+ * Halo update of all six tiles of batched storages (the scalar and vector
+ * halo updaters of fv3gfs-util used at fv3core/stencils/dyn_core.py:216-286;
+ * no counterpart code in the reference itself).  map: device array of
+ * nentries x 8 int32 (dst_tile, dst_i, dst_j, dst_comp, src_tile, src_i,
+ * src_j, src_comp*2 + negate), built by fv3core_amd/halo.py.  fields: host
+ * pointer to 16 device pointers; nfields of them are used, in groups of
+ * ncomp (1 scalar, 2 = x/y vector components).  d->ntiles must be 6.
+ */
+int32_t fv3_halo_update_f64(const fv3_domain* d, int32_t nentries, const int32_t* map,
+                            int32_t nfields, int32_t ncomp, const void* fields, void* stream);
+int32_t fv3_halo_update_f32(const fv3_domain* d, int32_t nentries, const int32_t* map,
+                            int32_t nfields, int32_t ncomp, const void* fields, void* stream);
+
+/*
+ * TracerAdvection.__call__ helpers (fv3core/stencils/tracer_2d_1l.py:199-262):
+ * flux_compute (x/y area fluxes from cxd/cyd on [isc, iec+1] x [jsc-3, jec+3]
+ * and [isc-3, iec+3] x [jsc, jec+1]) fused with cmax_multiply_by_frac
+ * (cxd, cyd, mfxd, mfyd and the area fluxes times 1/n_split when n_split > 1),
+ * over the full storage plus one point.
+ */
+int32_t fv3_tracer_flux_prep_f64(const fv3_domain* d, int32_t n_split, double* cxd,
+                                 double* cyd, double* mfxd, double* mfyd, double* xfx,
+                                 double* yfx, const double* dxa, const double* dya,
+                                 const double* dx, const double* dy, const double* sin_sg1,
+                                 const double* sin_sg2, const double* sin_sg3,
+                                 const double* sin_sg4, void* stream);
+/*
+ * dp_fluxadjustment (mode 0: out = dp2 = dp1 + div(fx, fy) * rarea) and
+ * q_adjust (mode 1: out = q = (q*dp1 + div(fx, fy) * rarea) / dp2) on the
+ * compute domain (tracer_2d_1l.py:92-117).
+ */
+int32_t fv3_tracer_adjust_f64(const fv3_domain* d, int32_t mode, double* out,
+                              const double* dp1, const double* fx, const double* fy,
+                              const double* rarea, const double* dp2, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
