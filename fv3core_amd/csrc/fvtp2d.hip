@@ -23,6 +23,120 @@ __device__ __forceinline__ T ld2(const T* __restrict__ p, int i, int j, int64_t 
   return (i >= 0 && i < ni && j >= 0 && j < nj) ? p[(int64_t)j * sj + i] : T(0);
 }
 
+// Number of XCDs on MI355X: consecutive workgroups are dispatched round-robin
+// over them, each with its own L2.
+constexpr int kNumXcd = 8;
+
+// Workgroup id -> logical block so that each XCD runs a contiguous range of
+// logical blocks (whole planes of one level), keeping the halo rows that
+// neighbouring blocks share in that XCD's L2.
+__device__ __forceinline__ int xcd_logical_block(int hw, int total) {
+  const int per = (total + kNumXcd - 1) / kNumXcd;
+  return (hw % kNumXcd) * per + hw / kNumXcd;
+}
+
+template <typename T, int ORD, bool EDGES>
+__device__ __forceinline__ void fvtp2d_block(
+    const Dom& d, int i0, int j0, bool same_q, const T* __restrict__ qx,
+    const T* __restrict__ qy, const T* __restrict__ cx, const T* __restrict__ cy,
+    const T* __restrict__ ax, const T* __restrict__ ay, const T* __restrict__ ux,
+    const T* __restrict__ uy, const T* __restrict__ ar, const T* __restrict__ dx,
+    const T* __restrict__ dy, T* __restrict__ fx, T* __restrict__ fy,
+    T (*__restrict__ sqx)[TR_TI + 6], T (*__restrict__ sqy_own)[TR_TI + 6],
+    T (*__restrict__ sym)[TR_TI + 5], T (*__restrict__ sxm)[TR_TI + 1],
+    T (*__restrict__ sady)[TR_TI + 5], T (*__restrict__ sadx)[TR_TI]) {
+  constexpr int TI = TR_TI, TJ = TR_TJ;
+  constexpr int QW = TI + 6, QH = TJ + 6;
+  const int tid = threadIdx.x;
+  const int ni = d.ni, nj = d.nj;
+  const int sj = (int)d.sj, sj2 = (int)d.sj2;
+  T (*sqy)[QW] = same_q ? sqx : sqy_own;
+  auto ld = [&](const T* __restrict__ p, int i, int j, int stride) -> T {
+    return (i >= 0 && i < ni && j >= 0 && j < nj) ? p[j * stride + i] : T(0);
+  };
+
+  // 1. stage q_x (and q_y unless it is the same storage) with a 3-cell halo
+  for (int n = tid; n < QW * QH; n += TR_NT) {
+    const int li = n % QW, lj = n / QW;
+    const int i = i0 - 3 + li, j = j0 - 3 + lj;
+    sqx[lj][li] = ld(qx, i, j, sj);
+    if (!same_q) sqy_own[lj][li] = ld(qy, i, j, sj);
+  }
+  __syncthreads();
+
+  // 2a. x-inner PPM on q_x: faces i in [i0, i0+TI], rows j in [j0-3, j0+TJ+2)
+  for (int n = tid; n < (TI + 1) * (TJ + 5); n += TR_NT) {
+    const int li = n % (TI + 1), lj = n / (TI + 1);
+    const int i = i0 + li, j = j0 - 3 + lj;
+    T q6[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) q6[m] = sqx[lj][li + m];
+    const T c = ld(cx, i, j, sj);
+    DxRow<T> dr{dx + (int64_t)j * sj2 + (i - 3), 1};
+    sxm[lj][li] = (j < nj) ? ppm_face_sel<T, ORD, EDGES>(q6, c, i, d.is, d.ie, dr) : T(0);
+  }
+  // 2b. y-inner PPM on q_y: faces j in [j0, j0+TJ], columns i in [i0-3, i0+TI+2)
+  for (int n = tid; n < (TI + 5) * (TJ + 1); n += TR_NT) {
+    const int li = n % (TI + 5), lj = n / (TI + 5);
+    const int i = i0 - 3 + li, j = j0 + lj;
+    T q6[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) q6[m] = sqy[lj + m][li];
+    const T c = ld(cy, i, j, sj);
+    DxRow<T> dr{dy + (int64_t)(j - 3) * sj2 + i, sj2};
+    sym[lj][li] = (i < ni) ? ppm_face_sel<T, ORD, EDGES>(q6, c, j, d.js, d.je, dr) : T(0);
+  }
+  __syncthreads();
+
+  // 3a. q_j (fvtp2d.py:50-60): cells i in [i0, i0+TI), rows [j0-3, j0+TJ+2)
+  for (int n = tid; n < TI * (TJ + 5); n += TR_NT) {
+    const int li = n % TI, lj = n / TI;
+    const int i = i0 + li, j = j0 - 3 + lj;
+    const T a = ld(ar, i, j, sj2);
+    const T f0 = ld(ax, i, j, sj);
+    const T f1 = ld(ax, i + 1, j, sj);
+    const T fx1_0 = f0 * sxm[lj][li];
+    const T fx1_1 = f1 * sxm[lj][li + 1];
+    const T area_with_x_flux = a + f0 - f1;
+    sadx[lj][li] = (sqx[lj][li + 3] * a + fx1_0 - fx1_1) / area_with_x_flux;
+  }
+  // 3b. q_i (fvtp2d.py:33-47): cells i in [i0-3, i0+TI+2), rows [j0, j0+TJ)
+  for (int n = tid; n < (TI + 5) * TJ; n += TR_NT) {
+    const int li = n % (TI + 5), lj = n / (TI + 5);
+    const int i = i0 - 3 + li, j = j0 + lj;
+    const T a = ld(ar, i, j, sj2);
+    const T g0 = ld(ay, i, j, sj);
+    const T g1 = ld(ay, i, j + 1, sj);
+    const T fyy0 = g0 * sym[lj][li];
+    const T fyy1 = g1 * sym[lj + 1][li];
+    sady[lj][li] = (sqy[lj + 3][li] * a + fyy0 - fyy1) / (a + g0 - g1);
+  }
+  __syncthreads();
+
+  // 4. outer sweeps and final_fluxes (fvtp2d.py:63-92)
+  for (int n = tid; n < TI * TJ; n += TR_NT) {
+    const int li = n % TI, lj = n / TI;
+    const int i = i0 + li, j = j0 + lj;
+    const int o = j * sj + i;
+    if (i <= d.iec + 1 && j <= d.jec) {
+      T q6[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) q6[m] = sady[lj][li + m];
+      DxRow<T> dr{dx + (int64_t)j * sj2 + (i - 3), 1};
+      const T outer = ppm_face_sel<T, ORD, EDGES>(q6, cx[o], i, d.is, d.ie, dr);
+      fx[o] = T(0.5) * (outer + sxm[lj + 3][li]) * ux[o];
+    }
+    if (i <= d.iec && j <= d.jec + 1) {
+      T q6[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) q6[m] = sadx[lj + m][li];
+      DxRow<T> dr{dy + (int64_t)(j - 3) * sj2 + i, sj2};
+      const T outer = ppm_face_sel<T, ORD, EDGES>(q6, cy[o], j, d.js, d.je, dr);
+      fy[o] = T(0.5) * (outer + sym[lj][li + 3]) * uy[o];
+    }
+  }
+}
+
 template <typename T, int ORD>
 __global__ void __launch_bounds__(TR_NT)
     fvtp2d_kernel(Dom d, const T* __restrict__ q_x, const T* __restrict__ q_y,
@@ -31,7 +145,7 @@ __global__ void __launch_bounds__(TR_NT)
                   const T* __restrict__ xmf, const T* __restrict__ ymf,
                   const T* __restrict__ area, const T* __restrict__ dxa,
                   const T* __restrict__ dya, T* __restrict__ fx, T* __restrict__ fy,
-                  int nbi) {
+                  int nbi, int nbj) {
   constexpr int TI = TR_TI, TJ = TR_TJ;
   constexpr int QW = TI + 6, QH = TJ + 6;
   __shared__ T sqx[QH][QW];
@@ -41,18 +155,19 @@ __global__ void __launch_bounds__(TR_NT)
   __shared__ T sady[TJ][TI + 5];      // q advected in y, cells (i0-3.., j0..)
   __shared__ T sadx[TJ + 5][TI];      // q advected in x, cells (i0.., j0-3..)
 
-  const int tid = threadIdx.x;
-  const int bi = blockIdx.x % nbi;
-  const int bj = blockIdx.x / nbi;
-  const int k = blockIdx.y;
-  const int t = blockIdx.z;
+  const int per_plane = nbi * nbj;
+  const int total = per_plane * d.nk * d.ntiles;
+  const int lb = xcd_logical_block(blockIdx.x, total);
+  if (lb >= total) return;
+  const int plane = lb / per_plane;
+  const int within = lb % per_plane;
+  const int bi = within % nbi, bj = within / nbi;
+  const int k = plane % d.nk, t = plane / d.nk;
   const int i0 = d.isc + bi * TI;
   const int j0 = d.jsc + bj * TJ;
   const int64_t off3 = (int64_t)t * d.st + (int64_t)k * d.sk;
   const int64_t off2 = (int64_t)t * d.st2;
-  const int ni = d.ni, nj = d.nj;
-  const int64_t sj = d.sj, sj2 = d.sj2;
-
+  const bool same_q = (q_x == q_y);
   const T* qx = q_x + off3;
   const T* qy = q_y + off3;
   const T* cx = crx + off3;
@@ -64,87 +179,19 @@ __global__ void __launch_bounds__(TR_NT)
   const T* ar = area + off2;
   const T* dx = dxa + off2;
   const T* dy = dya + off2;
-
-  // 1. stage q_x and q_y with a 3-cell halo
-  for (int n = tid; n < QW * QH; n += TR_NT) {
-    const int li = n % QW, lj = n / QW;
-    const int i = i0 - 3 + li, j = j0 - 3 + lj;
-    sqx[lj][li] = ld2(qx, i, j, sj, ni, nj);
-    sqy[lj][li] = ld2(qy, i, j, sj, ni, nj);
-  }
-  __syncthreads();
-
-  // 2a. x-inner PPM on q_x: faces i in [i0, i0+TI], rows j in [j0-3, j0+TJ+2)
-  for (int n = tid; n < (TI + 1) * (TJ + 5); n += TR_NT) {
-    const int li = n % (TI + 1), lj = n / (TI + 1);
-    const int i = i0 + li, j = j0 - 3 + lj;
-    T q6[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) q6[m] = sqx[lj][li + m];
-    const T c = ld2(cx, i, j, sj, ni, nj);
-    DxRow<T> dr{dx + (int64_t)j * sj2 + (i - 3), 1};
-    sxm[lj][li] = (j >= 0 && j < nj) ? ppm_face<T, ORD>(q6, c, i, d.is, d.ie, dr) : T(0);
-  }
-  // 2b. y-inner PPM on q_y: faces j in [j0, j0+TJ], columns i in [i0-3, i0+TI+2)
-  for (int n = tid; n < (TI + 5) * (TJ + 1); n += TR_NT) {
-    const int li = n % (TI + 5), lj = n / (TI + 5);
-    const int i = i0 - 3 + li, j = j0 + lj;
-    T q6[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) q6[m] = sqy[lj + m][li];
-    const T c = ld2(cy, i, j, sj, ni, nj);
-    DxRow<T> dr{dy + (int64_t)(j - 3) * sj2 + i, sj2};
-    sym[lj][li] = (i >= 0 && i < ni) ? ppm_face<T, ORD>(q6, c, j, d.js, d.je, dr) : T(0);
-  }
-  __syncthreads();
-
-  // 3a. q_j (fvtp2d.py:50-60): cells i in [i0, i0+TI), rows [j0-3, j0+TJ+2)
-  for (int n = tid; n < TI * (TJ + 5); n += TR_NT) {
-    const int li = n % TI, lj = n / TI;
-    const int i = i0 + li, j = j0 - 3 + lj;
-    const T a = ld2(ar, i, j, sj2, ni, nj);
-    const T f0 = ld2(ax, i, j, sj, ni, nj);
-    const T f1 = ld2(ax, i + 1, j, sj, ni, nj);
-    const T fx1_0 = f0 * sxm[lj][li];
-    const T fx1_1 = f1 * sxm[lj][li + 1];
-    const T area_with_x_flux = a + f0 - f1;
-    sadx[lj][li] = (sqx[lj][li + 3] * a + fx1_0 - fx1_1) / area_with_x_flux;
-  }
-  // 3b. q_i (fvtp2d.py:33-47): cells i in [i0-3, i0+TI+2), rows [j0, j0+TJ)
-  for (int n = tid; n < (TI + 5) * TJ; n += TR_NT) {
-    const int li = n % (TI + 5), lj = n / (TI + 5);
-    const int i = i0 - 3 + li, j = j0 + lj;
-    const T a = ld2(ar, i, j, sj2, ni, nj);
-    const T g0 = ld2(ay, i, j, sj, ni, nj);
-    const T g1 = ld2(ay, i, j + 1, sj, ni, nj);
-    const T fyy0 = g0 * sym[lj][li];
-    const T fyy1 = g1 * sym[lj + 1][li];
-    sady[lj][li] = (sqy[lj + 3][li] * a + fyy0 - fyy1) / (a + g0 - g1);
-  }
-  __syncthreads();
-
-  // 4. outer sweeps and final_fluxes (fvtp2d.py:63-92)
-  for (int n = tid; n < TI * TJ; n += TR_NT) {
-    const int li = n % TI, lj = n / TI;
-    const int i = i0 + li, j = j0 + lj;
-    if (i <= d.iec + 1 && j <= d.jec) {
-      T q6[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) q6[m] = sady[lj][li + m];
-      const int64_t o = (int64_t)j * sj + i;
-      DxRow<T> dr{dx + (int64_t)j * sj2 + (i - 3), 1};
-      const T outer = ppm_face<T, ORD>(q6, cx[o], i, d.is, d.ie, dr);
-      fx[off3 + o] = T(0.5) * (outer + sxm[lj + 3][li]) * ux[o];
-    }
-    if (i <= d.iec && j <= d.jec + 1) {
-      T q6[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) q6[m] = sadx[lj + m][li];
-      const int64_t o = (int64_t)j * sj + i;
-      DxRow<T> dr{dy + (int64_t)(j - 3) * sj2 + i, sj2};
-      const T outer = ppm_face<T, ORD>(q6, cy[o], j, d.js, d.je, dr);
-      fy[off3 + o] = T(0.5) * (outer + sym[lj][li + 3]) * uy[o];
-    }
+  T* fxp = fx + off3;
+  T* fyp = fy + off3;
+  // a block needs the tile-edge formulas when its stencil reach (cells
+  // i0-4 .. i0+TI+4) touches the edge regions (anchor -2 .. +3)
+  auto near = [](int lo, int hi, int a) { return hi >= a - 2 && lo <= a + 3; };
+  const bool edges = near(i0 - 4, i0 + TI + 4, d.is) || near(i0 - 4, i0 + TI + 4, d.ie) ||
+                     near(j0 - 4, j0 + TJ + 4, d.js) || near(j0 - 4, j0 + TJ + 4, d.je);
+  if (edges) {
+    fvtp2d_block<T, ORD, true>(d, i0, j0, same_q, qx, qy, cx, cy, ax, ay, ux, uy, ar, dx, dy,
+                               fxp, fyp, sqx, sqy, sym, sxm, sady, sadx);
+  } else {
+    fvtp2d_block<T, ORD, false>(d, i0, j0, same_q, qx, qy, cx, cy, ax, ay, ux, uy, ar, dx, dy,
+                                fxp, fyp, sqx, sqy, sym, sxm, sady, sadx);
   }
 }
 
@@ -161,28 +208,32 @@ static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T*
   FV3_REQUIRE(hord == 5 || hord == 6 || hord == 7 || hord == 8, FV3_E_UNSUPPORTED,
               "%s: hord=%d not supported (5, 6, 7, 8)", who, hord);
   if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
-  FV3_REQUIRE(dom->nk <= 65535 && dom->ntiles <= 65535, FV3_E_ARG, "%s: grid too large", who);
+  FV3_REQUIRE(dom->sj < (1 << 30) && dom->sj2 < (1 << 30), FV3_E_ARG, "%s: row stride too large",
+              who);
   Dom d = to_dom(dom);
   const int nfi = d.iec + 2 - d.isc, nfj = d.jec + 2 - d.jsc;
   const int nbi = (nfi + TR_TI - 1) / TR_TI, nbj = (nfj + TR_TJ - 1) / TR_TJ;
-  dim3 grid(nbi * nbj, d.nk, d.ntiles), block(TR_NT);
+  const int64_t total = (int64_t)nbi * nbj * d.nk * d.ntiles;
+  FV3_REQUIRE(total < (int64_t)1 << 30, FV3_E_ARG, "%s: grid too large", who);
+  const int64_t launched = (total + kNumXcd - 1) / kNumXcd * kNumXcd;
+  dim3 grid((unsigned)launched), block(TR_NT);
   hipStream_t s = as_stream(stream);
   switch (hord) {
     case 5:
       hipLaunchKernelGGL((fvtp2d_kernel<T, 5>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi);
+                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
       break;
     case 6:
       hipLaunchKernelGGL((fvtp2d_kernel<T, 6>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi);
+                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
       break;
     case 7:
       hipLaunchKernelGGL((fvtp2d_kernel<T, 7>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi);
+                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
       break;
     default:
       hipLaunchKernelGGL((fvtp2d_kernel<T, 8>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi);
+                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
       break;
   }
   FV3_HIP_CHECK_LAUNCH(who);

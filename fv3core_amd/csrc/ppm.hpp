@@ -81,9 +81,10 @@ template <typename T>
 struct DxRow {
   const T* base;  // points at dx of cell i-3
   int64_t stride;
+  int count = 6;  // entries that may be read (the rest are filled with 1)
   __device__ __forceinline__ void load(T (&d)[6]) const {
 #pragma unroll
-    for (int n = 0; n < 6; ++n) d[n] = base[n * stride];
+    for (int n = 0; n < 6; ++n) d[n] = n < count ? base[n * stride] : T(1);
   }
 };
 
@@ -136,6 +137,18 @@ __device__ __forceinline__ void ppm_blbr8(const T* q, int p, int m, int is, int 
     br = xbr - q[p];
     ppm_standard_constraint(bl, br);
   }
+}
+
+
+// ppm_blbr8 on a 5-cell window w[0..4] centred on the cell (w[2]); dxr points
+// at dx of the window's first cell.
+template <typename T>
+__device__ __forceinline__ void ppm_blbr8_w(const T (&w)[5], int m, int is, int ie,
+                                            const DxRow<T>& dxr, T& bl, T& br) {
+  T q[6] = {w[0], w[1], w[2], w[3], w[4], w[4]};
+  // ppm_blbr8 reads q[p-2..p+2] and loads dx[0..5] relative to its own
+  // origin; with p = 2 the window's first cell is the origin
+  ppm_blbr8(q, 2, m, is, ie, dxr, bl, br);
 }
 
 // Interface value al at face f whose left cell is q[p-1] (hord < 8,
@@ -207,6 +220,71 @@ __device__ __forceinline__ T ppm_face(const T (&q6)[6], T c, int i, int is, int 
       T fx1 = (T(1) + c) * (bl0 + c * b00);
       return q6[3] + fx1 * mask;
     }
+  }
+}
+
+
+// Branch-free form of ppm_face: the upwind cell's 5-point window is selected
+// first (lanes of a wave may differ in the sign of c), then one straight-line
+// evaluation follows.  Arithmetic and rounding are those of ppm_face.  With
+// EDGES = false the tile-edge regions are not tested (blocks far from any
+// tile edge).
+template <typename T, int ORD, bool EDGES>
+__device__ __forceinline__ T ppm_face_sel(const T (&q6)[6], T c, int i, int is, int ie,
+                                          const DxRow<T>& dxr) {
+  const bool pos = c > T(0);
+  if constexpr (ORD == 8) {
+    T w[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) w[m] = pos ? q6[m] : q6[m + 1];
+    T bl, br;
+    if constexpr (EDGES) {
+      DxRow<T> dw{dxr.base + (pos ? 0 : dxr.stride), dxr.stride, 5};
+      ppm_blbr8_w(w, pos ? i - 1 : i, is, ie, dw, bl, br);
+    } else {
+      const T dmm = ppm_dm(w[0], w[1], w[2]);
+      const T dm0 = ppm_dm(w[1], w[2], w[3]);
+      const T dmp = ppm_dm(w[2], w[3], w[4]);
+      const T third = T(1.0 / 3.0);
+      const T alL = T(0.5) * (w[1] + w[2]) + third * (dmm - dm0);
+      const T alR = T(0.5) * (w[2] + w[3]) + third * (dm0 - dmp);
+      const T xt = T(2) * dm0;
+      bl = -sgn(fmin(fabs(xt), fabs(alL - w[2])), xt);
+      br = sgn(fmin(fabs(xt), fabs(alR - w[2])), xt);
+    }
+    const T b0 = bl + br;
+    const T fpos = (T(1) - c) * (br - c * b0);
+    const T fneg = (T(1) + c) * (bl + c * b0);
+    return w[2] + (pos ? fpos : fneg);
+  } else {
+    T alm, al0, alp;
+    if constexpr (EDGES) {
+      alm = ppm_al(q6, 2, i - 1, is, ie, dxr);
+      al0 = ppm_al(q6, 3, i, is, ie, dxr);
+      alp = ppm_al(q6, 4, i + 1, is, ie, dxr);
+    } else {
+      alm = T(FV3_PPM_P1) * (q6[1] + q6[2]) + T(FV3_PPM_P2) * (q6[0] + q6[3]);
+      al0 = T(FV3_PPM_P1) * (q6[2] + q6[3]) + T(FV3_PPM_P2) * (q6[1] + q6[4]);
+      alp = T(FV3_PPM_P1) * (q6[3] + q6[4]) + T(FV3_PPM_P2) * (q6[2] + q6[5]);
+    }
+    const T blm = alm - q6[2];
+    const T brm = al0 - q6[2];
+    const T b0m = blm + brm;
+    const T bl0 = al0 - q6[3];
+    const T br0 = alp - q6[3];
+    const T b00 = bl0 + br0;
+    bool smt_m, smt_0;
+    if constexpr (ORD == 5) {
+      smt_m = blm * brm < T(0);
+      smt_0 = bl0 * br0 < T(0);
+    } else {
+      smt_m = (T(3) * fabs(b0m)) < fabs(blm - brm);
+      smt_0 = (T(3) * fabs(b00)) < fabs(bl0 - br0);
+    }
+    const T mask = (smt_m || smt_0) ? T(1) : T(0);
+    const T fpos = q6[2] + (T(1) - c) * (brm - c * b0m) * mask;
+    const T fneg = q6[3] + (T(1) + c) * (bl0 + c * b00) * mask;
+    return pos ? fpos : fneg;
   }
 }
 
