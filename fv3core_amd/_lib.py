@@ -203,3 +203,46 @@ def fn(name: str, dtype: torch.dtype):
     if suffix is None:
         raise TypeError(f"unsupported dtype {dtype}")
     return getattr(lib(), name + suffix)
+
+
+METRIC_NAMES = (
+    "dx", "dy", "dxa", "dya", "dxc", "dyc",
+    "rdx", "rdy", "rdxa", "rdya", "rdxc", "rdyc",
+    "area", "rarea", "area_c", "rarea_c",
+    "sin_sg1", "sin_sg2", "sin_sg3", "sin_sg4",
+    "cos_sg1", "cos_sg2", "cos_sg3", "cos_sg4",
+    "cosa", "sina", "cosa_u", "cosa_v", "cosa_s", "sina_u", "sina_v",
+    "rsin_u", "rsin_v", "rsina", "rsin2",
+    "fC", "f0", "del6_u", "del6_v", "divg_u", "divg_v",
+    "agrid1", "agrid2", "bgrid1", "bgrid2",
+    "edge_w", "edge_e", "edge_s", "edge_n",
+)
+
+
+class Metrics(ctypes.Structure):
+    """Mirror of ``fv3_metrics`` in include/fv3core_amd.h."""
+
+    _fields_ = [(n, ctypes.c_void_p) for n in METRIC_NAMES]
+
+
+def metrics_struct(grid_data, dtype: torch.dtype, names: Sequence[str]):
+    """(Metrics, ref2) for the named GridData fields, checked for one layout.
+
+    Every named field must exist, be a device tensor of ``dtype`` and share
+    the first field's IJ layout; the others stay NULL.
+    """
+    m = Metrics()
+    ref2 = None
+    for n in names:
+        if not grid_data.has(n):
+            raise ValueError(f"GridData lacks the metric term {n!r}")
+        t = getattr(grid_data, n)
+        require_device(t)
+        if t.dtype != dtype:
+            raise TypeError(f"metric {n} is {t.dtype}, the call computes in {dtype}")
+        if ref2 is None:
+            ref2 = t
+        elif layout2(t)[1:] != layout2(ref2)[1:] or t.shape != ref2.shape:
+            raise ValueError(f"metric {n} does not share the IJ layout of {names[0]}")
+        setattr(m, n, t.data_ptr())
+    return m, ref2

@@ -117,6 +117,34 @@ def uniform_metric(n: int, n_halo: int = 3, dx: float = 1.0e5):
     )
 
 
+def plane_metric(n: int, n_halo: int = 3, dx: float = 1.0e5, f: float = 0.0):
+    """Every IJ metric term of an orthogonal uniform tile (a plane).
+
+    sin_sg = 1, cos_sg = 0, cosa = 0, sina = 1 and all spacings ``dx``; the
+    Coriolis parameter fC is the constant ``f``.
+    """
+    m = uniform_metric(n, n_halo, dx)
+    ones = np.ones_like(m["area"])
+    for k in (1, 2, 3, 4):
+        m[f"sin_sg{k}"] = ones.copy()
+        m[f"cos_sg{k}"] = 0.0 * ones
+    for name in ("cosa", "cosa_u", "cosa_v", "cosa_s"):
+        m[name] = 0.0 * ones
+    for name in ("sina", "sina_u", "sina_v", "rsina", "rsin_u", "rsin_v", "rsin2"):
+        m[name] = ones.copy()
+    m["area_c"] = dx * dx * ones
+    m["rarea_c"] = 1.0 / m["area_c"]
+    for r, b in (("rdx", "dx"), ("rdy", "dy"), ("rdxc", "dxc"), ("rdyc", "dyc"),
+                 ("rdxa", "dxa"), ("rdya", "dya")):
+        m[r] = 1.0 / m[b]
+    m["divg_u"] = ones.copy()
+    m["divg_v"] = ones.copy()
+    m["fC"] = f * ones
+    m["f0"] = f * ones
+    m["da_min_c"] = float(dx * dx)
+    return m
+
+
 def hybrid_coefficients(km: int = 79):
     """(ak, bk, ptop) of the reference's 79-layer hybrid grid (fv3core/grid/eta.py:38-217)."""
     if km != 79:

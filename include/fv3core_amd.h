@@ -208,6 +208,53 @@ int32_t fv3_tracer_adjust_f64(const fv3_domain* d, int32_t mode, double* out,
                               const double* dp1, const double* fx, const double* fy,
                               const double* rarea, const double* dp2, void* stream);
 
+/*
+ * IJ metric terms of GridData (fv3core/utils/grid.py:474-798), as device
+ * pointers of the entry point's floating type, each laid out with the
+ * domain's sj2/st2 strides.  Members an entry point does not read may be
+ * NULL; each entry point checks the ones it reads.  agrid1/agrid2 and
+ * bgrid1/bgrid2 are the longitude/latitude of cell centres and corners and
+ * edge_w/e/s/n the edge weights of a2b_ord4 (a2b_ord4.py:474-490), stored
+ * as IJ arrays.
+ */
+typedef struct fv3_metrics {
+  const void *dx, *dy, *dxa, *dya, *dxc, *dyc;
+  const void *rdx, *rdy, *rdxa, *rdya, *rdxc, *rdyc;
+  const void *area, *rarea, *area_c, *rarea_c;
+  const void *sin_sg1, *sin_sg2, *sin_sg3, *sin_sg4;
+  const void *cos_sg1, *cos_sg2, *cos_sg3, *cos_sg4;
+  const void *cosa, *sina, *cosa_u, *cosa_v, *cosa_s, *sina_u, *sina_v;
+  const void *rsin_u, *rsin_v, *rsina, *rsin2;
+  const void *fC, *f0, *del6_u, *del6_v, *divg_u, *divg_v;
+  const void *agrid1, *agrid2, *bgrid1, *bgrid2;
+  const void *edge_w, *edge_e, *edge_s, *edge_n;
+} fv3_metrics;
+
+/*
+ * DGrid2AGrid2CGridVectors.__call__ (fv3core/stencils/d2a2c_vect.py:573-651):
+ * D-grid winds u, v -> A-grid ua, va and C-grid uc, vc, utc, vtc, with the
+ * tile-edge interpolations and corner fills.  work: 2 slabs of work_slab
+ * elements, each addressed with the 3-D strides (utmp, vtmp).
+ */
+int32_t fv3_d2a2c_vect_f64(const fv3_domain* d, const fv3_metrics* g, double* uc, double* vc,
+                           const double* u, const double* v, double* ua, double* va,
+                           double* utc, double* vtc, double* work, int64_t work_slab,
+                           void* stream);
+
+/*
+ * CGridShallowWaterDynamics.__call__ (fv3core/stencils/c_sw.py:534-659):
+ * advances the C-grid winds uc, vc by dt2 and returns the half-step delp and
+ * pt in delpc, ptc (zero outside [isc-1, iec+1]^2 in the full domain).  delp,
+ * pt and w get the reference's in-place corner fills; omga receives the
+ * transported w; divgd is written on the compute corners when nord > 0.
+ * work: 7 slabs (utmp, vtmp, fx, fx1, fx2, ke, vort).
+ */
+int32_t fv3_c_sw_f64(const fv3_domain* d, const fv3_metrics* g, int32_t nord, double dt2,
+                     double* delp, double* pt, const double* u, const double* v, double* w,
+                     double* uc, double* vc, double* ua, double* va, double* ut, double* vt,
+                     double* divgd, double* omga, double* delpc, double* ptc, double* work,
+                     int64_t work_slab, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
