@@ -222,6 +222,39 @@ def tile_metrics(n: int, n_halo: int = 3, radius: float = RADIUS) -> Dict[str, n
     agrid3[:, :nc, :nc] = C
     m["grid3"] = grid3
     m["agrid3"] = agrid3
+    # longitude / latitude of corners (bgrid) and centres (agrid)
+    m["bgrid1"] = np.arctan2(P[..., 1], P[..., 0])
+    m["bgrid2"] = np.arcsin(np.clip(P[..., 2], -1.0, 1.0))
+    ag1 = np.zeros(shp)
+    ag2 = np.zeros(shp)
+    ag1[:, :nc, :nc] = np.arctan2(C[..., 1], C[..., 0])
+    ag2[:, :nc, :nc] = np.arcsin(np.clip(C[..., 2], -1.0, 1.0))
+    m["agrid1"], m["agrid2"] = ag1, ag2
+    # a2b_ord4 edge weights (fv3core/grid/geometry.py:604-713): at a corner
+    # point on a tile edge, the weight of the along-edge neighbour behind it,
+    # d2 / (d1 + d2) with d1, d2 the distances to the midpoints of the A-grid
+    # pairs straddling the edge on either side.  Stored per IJ point.
+    for name in ("edge_w", "edge_e", "edge_s", "edge_n"):
+        m[name] = np.full(shp, 0.5)
+    for t in range(6):
+        for name, ia, ib, along_i in (("edge_w", h - 1, h, False), ("edge_e", h + n, h + n - 1, False),
+                                      ("edge_s", h - 1, h, True), ("edge_n", h + n, h + n - 1, True)):
+            ip = h if ia < ib else h + n  # corner-point index across the edge
+            for k in range(h + 1, h + n):
+                if along_i:
+                    mid0 = _normalize(C[t, k - 1, ia] + C[t, k - 1, ib])
+                    mid1 = _normalize(C[t, k, ia] + C[t, k, ib])
+                    pt = P[t, k, ip]
+                else:
+                    mid0 = _normalize(C[t, ia, k - 1] + C[t, ib, k - 1])
+                    mid1 = _normalize(C[t, ia, k] + C[t, ib, k])
+                    pt = P[t, ip, k]
+                d1 = _gc_dist(mid0, pt)
+                d2 = _gc_dist(mid1, pt)
+                if along_i:
+                    m[name][t, k, :] = d2 / (d1 + d2)
+                else:
+                    m[name][t, :, k] = d2 / (d1 + d2)
     comp = (slice(None), slice(h, h + n), slice(h, h + n))
     m["da_min"] = float(np.min(m["area"][comp]))
     m["da_max"] = float(np.max(m["area"][comp]))

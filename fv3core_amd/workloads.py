@@ -142,6 +142,14 @@ def plane_metric(n: int, n_halo: int = 3, dx: float = 1.0e5, f: float = 0.0):
     m["fC"] = f * ones
     m["f0"] = f * ones
     m["da_min_c"] = float(dx * dx)
+    # small-angle longitude/latitude of corners and centres near the equator
+    h = dx / 6.3712e6
+    ii = np.arange(ones.shape[0])[:, None] * ones
+    jj = np.arange(ones.shape[1])[None, :] * ones
+    m["bgrid1"], m["bgrid2"] = ii * h, jj * h
+    m["agrid1"], m["agrid2"] = (ii + 0.5) * h, (jj + 0.5) * h
+    for name in ("edge_w", "edge_e", "edge_s", "edge_n"):
+        m[name] = 0.5 * ones
     return m
 
 

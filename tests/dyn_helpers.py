@@ -44,6 +44,8 @@ class Case:
         else:
             m = plane_metric(n, f=f)
             self.metric = [{k: v for k, v in m.items() if isinstance(v, np.ndarray)}]
+        # scalar terms (da_min, da_min_c ...), the same for every tile
+        self.scalars = {k: float(v) for k, v in m.items() if not isinstance(v, np.ndarray)}
         self.shape = (n + 7, n + 7, nz + 1)
 
     def field(self, mean, amp):
