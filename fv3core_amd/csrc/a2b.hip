@@ -1,0 +1,256 @@
+// AGrid2BGridFourthOrder (a2b_ord4): cell-centre values to cell corners.
+//
+// Reference: fv3core/stencils/a2b_ord4.py:1-695.  Three launches:
+//   1 edges:  the four corner extrapolations (great-circle distances on
+//             the A/B-grid longitudes and latitudes) and the tile-edge
+//             lines qout_x_edge / qout_y_edge, into qout and the edge
+//             temporary;
+//   2 means:  ppm_volume_mean_x on [isc, iec+1] x [jsc-2, jec+2] and
+//             ppm_volume_mean_y on [isc-2, iec+2] x [jsc, jec+1];
+//   3 interp: a2b_interpolation on the corners off the tile-edge lines
+//             (and the optional copy back into qin).
+#include "kernels.hpp"
+
+namespace fv3 {
+
+namespace {
+
+constexpr double kA1 = 9.0 / 16.0;
+constexpr double kA2 = -1.0 / 16.0;
+constexpr double kB1 = 7.0 / 12.0;
+constexpr double kB2 = -1.0 / 12.0;
+constexpr double kCc1 = 2.0 / 3.0;
+constexpr double kCc2 = -1.0 / 6.0;
+
+// great_circle_dist (a2b_ord4.py:37-41)
+template <typename T>
+__device__ __forceinline__ T gc_dist(T p1a, T p1b, T p2a, T p2b) {
+  const T sb = sin((p1b - p2b) / T(2));
+  const T sa = sin((p1a - p2a) / T(2));
+  const T tb = sb * sb;
+  const T ta = sa * sa;
+  return asin(sqrt(tb + cos(p1b) * cos(p2b) * ta)) * T(2);
+}
+
+// extrap_corner (a2b_ord4.py:44-57) from the corner point o2c and the two
+// centres at (di1, dj1), (di2, dj2) relative to the point.
+template <typename T>
+__device__ __forceinline__ T extrap(const Met<T>& m, const Dom& d, const T* q, int64_t o,
+                                    int64_t o2, int di1, int dj1, int di2, int dj2) {
+  const int64_t a1 = o2 + di1 + (int64_t)dj1 * d.sj2, a2 = o2 + di2 + (int64_t)dj2 * d.sj2;
+  const T p0a = m.bgrid1[o2], p0b = m.bgrid2[o2];
+  const T x1 = gc_dist(m.agrid1[a1], m.agrid2[a1], p0a, p0b);
+  const T x2 = gc_dist(m.agrid1[a2], m.agrid2[a2], p0a, p0b);
+  const T qa = q[o + di1 + (int64_t)dj1 * d.sj], qb = q[o + di2 + (int64_t)dj2 * d.sj];
+  return qa + x1 / (x2 - x1) * (qa - qb);
+}
+
+template <typename T>
+__device__ __forceinline__ T qx_edge_west(const T* q, const T* dx) {
+  // q[], dx[] centred at the point (index 0 = the point)
+  const T g_in = dx[1] / dx[0];
+  const T g_ou = dx[-2] / dx[-1];
+  return T(0.5) * (((T(2) + g_in) * q[0] - q[1]) / (T(1) + g_in) +
+                   ((T(2) + g_ou) * q[-1] - q[-2]) / (T(1) + g_ou));
+}
+
+template <typename T>
+__device__ __forceinline__ T qx_edge_west2(const T* q, const T* dx) {
+  const T g_in = dx[0] / dx[-1];
+  const T g_ou = dx[-3] / dx[-2];
+  const T left = T(0.5) * (((T(2) + g_in) * q[-1] - q[0]) / (T(1) + g_in) +
+                           ((T(2) + g_ou) * q[-2] - q[-3]) / (T(1) + g_ou));
+  const T right = T(kB2) * (q[-1] + q[2]) + T(kB1) * (q[0] + q[1]);
+  return (T(3) * (g_in * q[-1] + q[0]) - (g_in * left + right)) / (T(2) + T(2) * g_in);
+}
+
+template <typename T>
+__device__ __forceinline__ T qx_edge_east(const T* q, const T* dx) {
+  const T g_in = dx[-2] / dx[-1];
+  const T g_ou = dx[1] / dx[0];
+  return T(0.5) * (((T(2) + g_in) * q[-1] - q[-2]) / (T(1) + g_in) +
+                   ((T(2) + g_ou) * q[0] - q[1]) / (T(1) + g_ou));
+}
+
+template <typename T>
+__device__ __forceinline__ T qx_edge_east2(const T* q, const T* dx) {
+  const T g_in = dx[-1] / dx[0];
+  const T g_ou = dx[2] / dx[1];
+  const T right = T(0.5) * (((T(2) + g_in) * q[0] - q[-1]) / (T(1) + g_in) +
+                            ((T(2) + g_ou) * q[1] - q[2]) / (T(1) + g_ou));
+  const T left = T(kB2) * (q[-3] + q[0]) + T(kB1) * (q[-2] + q[-1]);
+  return (T(3) * (q[-1] + g_in * q[0]) - (g_in * right + left)) / (T(2) + T(2) * g_in);
+}
+
+// ppm_volume_mean_x / _y at a point: q7/dx7 hold cells p-3..p+3 along the
+// direction; a = absolute index of the point along it, s/e the anchors.
+template <typename T>
+__device__ __forceinline__ T volume_mean(const T (&q7)[7], const T (&dx7)[7], int a, int s,
+                                         int e) {
+  const T* q = q7 + 3;
+  const T* dx = dx7 + 3;
+  // regions in the reference's order: start, start+1, end+1, end (last wins)
+  if (a == e) return qx_edge_east2(q, dx);
+  if (a == e + 1) return qx_edge_east(q, dx);
+  if (a == s + 1) return qx_edge_west2(q, dx);
+  if (a == s) return qx_edge_west(q, dx);
+  return T(kB2) * (q[-2] + q[1]) + T(kB1) * (q[-1] + q[0]);
+}
+
+}  // namespace
+
+template <typename T>
+int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k0, int nk,
+                    const T* qin, T* qout, T* qx, T* qy, T* edges, int replace, T* qin_w,
+                    const char* who) {
+  if (nk <= 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const bool w = has_edge(dom->i_start), e = has_edge(dom->i_end);
+  const bool so = has_edge(dom->j_start), no = has_edge(dom->j_end);
+  const int js2 = so ? d.jsc + 1 : d.jsc, je1 = no ? d.jec : d.jec + 1;
+  const int is2 = w ? d.isc + 1 : d.isc, ie1 = e ? d.iec : d.iec + 1;
+  // 1: corners and edge lines, over the box [isc, iec+1]^2 restricted to them
+  const Box bx{d.isc, d.iec + 1, d.jsc, d.jec + 1};
+  launch_box(s, d, bx, k0, nk, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const int64_t o2 = at2(d, i, j, t);
+    T val;
+    const bool ci = i == d.isc || i == d.iec + 1, cj = j == d.jsc || j == d.jec + 1;
+    if (ci && cj) {
+      // _sw_corner at (isc, jsc), _nw_corner at (iec+1, jsc), _ne_corner at
+      // (iec+1, jec+1), _se_corner at (isc, jec+1) (a2b_ord4.py:617-658)
+      T e1, e2, e3;
+      if (i == d.isc && j == d.jsc) {
+        e1 = extrap(m, d, qin, o, o2, 0, 0, 1, 1);
+        e2 = extrap(m, d, qin, o, o2, -1, 0, -2, 1);
+        e3 = extrap(m, d, qin, o, o2, 0, -1, 1, -2);
+      } else if (j == d.jsc) {
+        e1 = extrap(m, d, qin, o, o2, -1, 0, -2, 1);
+        e2 = extrap(m, d, qin, o, o2, -1, -1, -2, -2);
+        e3 = extrap(m, d, qin, o, o2, 0, 0, 1, 1);
+      } else if (i == d.iec + 1) {
+        e1 = extrap(m, d, qin, o, o2, -1, -1, -2, -2);
+        e2 = extrap(m, d, qin, o, o2, 0, -1, 1, -2);
+        e3 = extrap(m, d, qin, o, o2, -1, 0, -2, 1);
+      } else {
+        e1 = extrap(m, d, qin, o, o2, 0, -1, 1, -2);
+        e2 = extrap(m, d, qin, o, o2, -1, -1, -2, -2);
+        e3 = extrap(m, d, qin, o, o2, 0, 0, 1, 1);
+      }
+      val = (e1 + e2 + e3) * (T(1) / T(3));
+    } else if (ci && ((i == d.isc && w) || (i == d.iec + 1 && e)) && j >= js2 && j <= je1) {
+      // qout_x_edge (a2b_ord4.py:313-323): q2 at j and j - 1
+      const T* dxa = m.dxa;
+      const T q2 = (qin[o - 1] * dxa[o2] + qin[o] * dxa[o2 - 1]) / (dxa[o2 - 1] + dxa[o2]);
+      const int64_t om = o - d.sj, om2 = o2 - d.sj2;
+      const T q2m = (qin[om - 1] * dxa[om2] + qin[om] * dxa[om2 - 1]) / (dxa[om2 - 1] + dxa[om2]);
+      const T wgt = (i == d.isc ? m.edge_w : m.edge_e)[o2];
+      val = wgt * q2m + (T(1) - wgt) * q2;
+    } else if (cj && ((j == d.jsc && so) || (j == d.jec + 1 && no)) && i >= is2 && i <= ie1) {
+      const T* dya = m.dya;
+      const int64_t sj = d.sj, sj2 = d.sj2;
+      const T q1 = (qin[o - sj] * dya[o2] + qin[o] * dya[o2 - sj2]) / (dya[o2 - sj2] + dya[o2]);
+      const T q1m = (qin[o - 1 - sj] * dya[o2 - 1] + qin[o - 1] * dya[o2 - 1 - sj2]) /
+                    (dya[o2 - 1 - sj2] + dya[o2 - 1]);
+      const T wgt = (j == d.jsc ? m.edge_s : m.edge_n)[o2];
+      val = wgt * q1m + (T(1) - wgt) * q1;
+    } else {
+      return;
+    }
+    qout[o] = val;
+    edges[o] = val;
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  // 2: volume means
+  const Box bm{d.isc - 2, d.iec + 2, d.jsc - 2, d.jec + 2};
+  launch_box(s, d, bm, k0, nk, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const int64_t o2 = at2(d, i, j, t);
+    if (i >= d.isc && i <= d.iec + 1) {
+      T q7[7], dx7[7];
+#pragma unroll
+      for (int n = 0; n < 7; ++n) {
+        q7[n] = qin[o + n - 3];
+        dx7[n] = m.dxa[o2 + n - 3];
+      }
+      qx[o] = volume_mean(q7, dx7, i, d.is, d.ie);
+    }
+    if (j >= d.jsc && j <= d.jec + 1) {
+      T q7[7], dy7[7];
+#pragma unroll
+      for (int n = 0; n < 7; ++n) {
+        q7[n] = qin[o + (int64_t)(n - 3) * d.sj];
+        dy7[n] = m.dya[o2 + (int64_t)(n - 3) * d.sj2];
+      }
+      qy[o] = volume_mean(q7, dy7, j, d.js, d.je);
+    }
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  // 3: interpolation off the tile-edge lines
+  const Box bi{w ? d.isc + 1 : d.isc, e ? d.iec : d.iec + 1, so ? d.jsc + 1 : d.jsc,
+               no ? d.jec : d.jec + 1};
+  launch_box(s, d, bi, k0, nk, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const int64_t sj = d.sj;
+    T qxx = T(kA2) * (qx[o - 2 * sj] + qx[o + sj]) + T(kA1) * (qx[o - sj] + qx[o]);
+    T qyy = T(kA2) * (qy[o - 2] + qy[o + 1]) + T(kA1) * (qy[o - 1] + qy[o]);
+    if (j == d.js + 1) {
+      const T up = T(kA2) * (qx[o - sj] + qx[o + 2 * sj]) + T(kA1) * (qx[o] + qx[o + sj]);
+      qxx = T(kCc1) * (qx[o - sj] + qx[o]) + T(kCc2) * (edges[o - sj] + up);
+    }
+    if (j == d.je) {
+      const T lo = T(kA2) * (qx[o - 3 * sj] + qx[o]) + T(kA1) * (qx[o - 2 * sj] + qx[o - sj]);
+      qxx = T(kCc1) * (qx[o - sj] + qx[o]) + T(kCc2) * (edges[o + sj] + lo);
+    }
+    if (i == d.is + 1) {
+      const T r = T(kA2) * (qy[o - 1] + qy[o + 2]) + T(kA1) * (qy[o] + qy[o + 1]);
+      qyy = T(kCc1) * (qy[o - 1] + qy[o]) + T(kCc2) * (edges[o - 1] + r);
+    }
+    if (i == d.ie) {
+      const T l = T(kA2) * (qy[o - 3] + qy[o]) + T(kA1) * (qy[o - 2] + qy[o - 1]);
+      qyy = T(kCc1) * (qy[o - 1] + qy[o]) + T(kCc2) * (edges[o + 1] + l);
+    }
+    qout[o] = T(0.5) * (qxx + qyy);
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  if (replace && qin_w) {
+    launch_box(s, d, bx, k0, nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      qin_w[o] = qout[o];
+    });
+    FV3_HIP_CHECK_LAUNCH(who);
+  }
+  return FV3_OK;
+}
+
+template int launch_a2b_ord4<double>(hipStream_t, const fv3_domain*, const Met<double>&, int, int,
+                                     const double*, double*, double*, double*, double*, int,
+                                     double*, const char*);
+
+int check_a2b_metrics(const fv3_metrics* g, const char* who) {
+  FV3_REQUIRE(g && g->dxa && g->dya && g->agrid1 && g->agrid2 && g->bgrid1 && g->bgrid2 &&
+                  g->edge_w && g->edge_e && g->edge_s && g->edge_n,
+              FV3_E_ARG, "%s: a2b_ord4 needs dxa, dya, agrid1/2, bgrid1/2 and edge_w/e/s/n",
+              who);
+  return FV3_OK;
+}
+
+}  // namespace fv3
+
+extern "C" int32_t fv3_a2b_ord4_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t replace,
+                                    double* qin, double* qout, double* work, int64_t work_slab,
+                                    void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_a2b_ord4";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  rc = check_a2b_metrics(g, who);
+  if (rc) return rc;
+  FV3_REQUIRE(qin && qout && work, FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)dom->nk * dom->sk,
+              FV3_E_ARG, "%s: work_slab too small", who);
+  const Met<double> m = to_met<double>(g);
+  return launch_a2b_ord4<double>(as_stream(stream), dom, m, 0, dom->nk, qin, qout, work,
+                                 work + work_slab, work + 2 * work_slab, replace, qin, who);
+}

@@ -255,6 +255,57 @@ int32_t fv3_c_sw_f64(const fv3_domain* d, const fv3_metrics* g, int32_t nord, do
                      double* divgd, double* omga, double* delpc, double* ptc, double* work,
                      int64_t work_slab, void* stream);
 
+/*
+ * AGrid2BGridFourthOrder.__call__ (fv3core/stencils/a2b_ord4.py:617-695):
+ * qout <- qin interpolated from cell centres to the corners [isc, iec+1]^2
+ * (and qin <- qout there when replace != 0).  work: 3 slabs (qx, qy, edge
+ * values).  Needs dxa, dya, agrid1/2, bgrid1/2 and edge_w/e/s/n.
+ */
+int32_t fv3_a2b_ord4_f64(const fv3_domain* d, const fv3_metrics* g, int32_t replace, double* qin,
+                         double* qout, double* work, int64_t work_slab, void* stream);
+
+/*
+ * Namelist values and per-level columns of DGridShallowWaterLagrangianDynamics
+ * (fv3core/stencils/d_sw.py:606-842, columns from get_column_namelist
+ * d_sw.py:512-563).  Column pointers are device arrays of d->nk entries.
+ */
+typedef struct fv3_dsw_params {
+  int32_t hord_dp, hord_tm, hord_mt, hord_vt;
+  int32_t hydrostatic;                         /* nonzero: no w transport */
+  int32_t nord_v4[4], nord_w4[4], nord_t4[4];  /* nord_v/w/t for k = 0, 1, 2, >= 3 */
+  int32_t div_k0;                              /* first level with nord > 0 */
+  int32_t div_nord;                            /* divergence-damping order there */
+  double dddmp, da_min_c;
+  double dd8;                                  /* (da_min_c * d4_bg) ** (div_nord + 1) */
+  double d_con;                                /* namelist d_con */
+  const double* damp_vt_deln;  /* DelnFlux damping of delp and pt (calc_damp, da_min) */
+  const double* damp_t_deln;   /* DelnFlux damping of q_con */
+  const double* damp_w_nosg;   /* DelnFluxNoSG damping of w (calc_damp, da_min_c) */
+  const double* damp_v_nosg;   /* DelnFluxNoSG damping of the vorticity */
+  const double* damp_w;        /* columns damp_w, damp_vt, ke_bg, d_con, d2_divg */
+  const double* damp_vt;
+  const double* ke_bg;
+  const double* d_con_col;
+  const double* d2_divg;
+} fv3_dsw_params;
+
+/* The storages d_sw reads and updates (same layout, nk + 1 levels each). */
+typedef struct fv3_dsw_state {
+  double *delpc, *delp, *ptc, *pt, *u, *v, *w, *uc, *vc, *ua, *va, *divgd;
+  double *mfx, *mfy, *cx, *cy, *crx, *cry, *xfx, *yfx, *q_con, *zh, *heat_source, *diss_est;
+} fv3_dsw_state;
+
+/*
+ * DGridShallowWaterLagrangianDynamics.__call__ (fv3core/stencils/d_sw.py:843-1131):
+ * one full D-grid step of dt on levels [0, d->nk) (transport on nk + 1).
+ * Writes the reference's in-place updates of every state member (uc, vc,
+ * delpc and ptc are used as scratch by the divergence damping, as there).
+ * work: 22 slabs of work_slab elements laid out like the fields (nk + 1 levels).
+ */
+int32_t fv3_d_sw_f64(const fv3_domain* d, const fv3_metrics* g, const fv3_dsw_params* p,
+                     const fv3_dsw_state* st, double dt, double* work, int64_t work_slab,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif
