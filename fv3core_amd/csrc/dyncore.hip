@@ -1,0 +1,655 @@
+// The acoustic step's remaining stencils: geopotential on the C grid
+// (updatedzc), height on the D grid (updatedzd), the nonhydrostatic
+// pressure gradient (nh_p_grad), the halo pressures (pk3_halo, pe_halo), the
+// C-grid pressure gradient (p_grad_c), Rayleigh damping (ray_fast),
+// del2cubed hyperdiffusion, the heating adjustment of pt and the small
+// column stencils of dyn_core.
+//
+// Reference: fv3core/stencils/updatedzc.py, updatedzd.py, nh_p_grad.py,
+// pk3_halo.py, pe_halo.py, ray_fast.py, del2cubed.py, temperature_adjust.py
+// and dyn_core.py:47-145.  Vertical recurrences run one column per lane.
+#include "kernels.hpp"
+
+namespace fv3 {
+
+namespace {
+
+constexpr double kDzMin = 2.0;
+
+// Value of the 2-cell corner-filled copy (fill_corners_2cells_x/_y,
+// corners.py:127-300) of q at (i, j): the rotated source for the 8
+// destinations of a tile's corners, q itself elsewhere.
+template <typename T>
+__device__ __forceinline__ T fill2_view(const T* q, const Dom& d, int i, int j, int64_t o,
+                                        bool ydir) {
+  const bool w = i == d.is - 1 || i == d.is - 2, e = i == d.ie + 1 || i == d.ie + 2;
+  const bool s = j == d.js - 1 || j == d.js - 2, n = j == d.je + 1 || j == d.je + 2;
+  if (!((w || e) && (s || n))) return q[o];
+  int di = 0, dj = 0;
+  if (!ydir) {
+    // destinations (is-1, js-1), (is-2, js-1) ... at rows js-1 / je+1
+    if (j != d.js - 1 && j != d.je + 1) return q[o];
+    const int a = w ? d.is - 1 - i : i - d.ie - 1;  // 0 or 1 cells beyond the corner column
+    const int sy = s ? 1 : -1;
+    di = w ? a : -a;
+    dj = sy * (1 + a);
+  } else {
+    if (i != d.is - 1 && i != d.ie + 1) return q[o];
+    const int b = s ? d.js - 1 - j : j - d.je - 1;
+    const int sx = w ? 1 : -1;
+    di = sx * (1 + b);
+    dj = s ? b : -b;
+  }
+  return q[o + di + (int64_t)dj * d.sj];
+}
+
+
+// ----------------------------------------------------------------- updatedzc
+template <typename T>
+__global__ void __launch_bounds__(256)
+    update_dz_c_kernel(Dom d, const T* __restrict__ dp0, const T* __restrict__ zs,
+                       const T* __restrict__ area, const T* __restrict__ ut,
+                       const T* __restrict__ vt, const T* __restrict__ gz, T* __restrict__ gzn,
+                       T* __restrict__ ws, T rdt) {
+  const int ni = d.iec - d.isc + 3, nj = d.jec - d.jsc + 3;
+  const int n = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (n >= ni * nj) return;
+  const int i = d.isc - 1 + n % ni, j = d.jsc - 1 + n / ni, t = (int)blockIdx.y;
+  const int64_t o2 = at2(d, i, j, t);
+  const int nk = d.nk;
+  const T a = area[o2];
+  // interface winds at (i, j), (i+1, j) for x and (i, j), (i, j+1) for y
+  auto iface = [&](const T* vel, int64_t oo, int k) -> T {
+    const int64_t sk = d.sk;
+    if (k == 0) {
+      const T ratio = dp0[0] / (dp0[0] + dp0[1]);
+      return vel[oo] + (vel[oo] - vel[oo + sk]) * ratio;
+    }
+    if (k == nk - 1) {
+      const T ratio = dp0[k - 1] / (dp0[k - 2] + dp0[k - 1]);
+      return vel[oo - sk] + (vel[oo - sk] - vel[oo - 2 * sk]) * ratio;
+    }
+    const T ir = T(1) / (dp0[k - 1] + dp0[k]);
+    return (dp0[k] * vel[oo - sk] + dp0[k - 1] * vel[oo]) * ir;
+  };
+  T last = T(0);
+  for (int k = 0; k < nk; ++k) {
+    const int64_t o = at3(d, i, j, k, t);
+    const T xfx0 = iface(ut, o, k), xfx1 = iface(ut, o + 1, k);
+    const T yfx0 = iface(vt, o, k), yfx1 = iface(vt, o + d.sj, k);
+    const T gx_m = fill2_view(gz, d, i - 1, j, o - 1, false);
+    const T gx_0 = fill2_view(gz, d, i, j, o, false);
+    const T gx_p = fill2_view(gz, d, i + 1, j, o + 1, false);
+    const T gy_m = fill2_view(gz, d, i, j - 1, o - d.sj, true);
+    const T gy_0 = fill2_view(gz, d, i, j, o, true);
+    const T gy_p = fill2_view(gz, d, i, j + 1, o + d.sj, true);
+    const T fx0 = xfx0 * (xfx0 > T(0) ? gx_m : gx_0);
+    const T fx1 = xfx1 * (xfx1 > T(0) ? gx_0 : gx_p);
+    const T fy0 = yfx0 * (yfx0 > T(0) ? gy_m : gy_0);
+    const T fy1 = yfx1 * (yfx1 > T(0) ? gy_0 : gy_p);
+    const T v = (gy_0 * a + fx0 - fx1 + fy0 - fy1) / (a + xfx0 - xfx1 + yfx0 - yfx1);
+    gzn[o] = v;
+    last = v;
+  }
+  ws[o2] = (zs[o2] - last) * rdt;
+  T above = last;
+  for (int k = nk - 2; k >= 0; --k) {
+    const int64_t o = at3(d, i, j, k, t);
+    const T kp1 = above + T(kDzMin);
+    T v = gzn[o];
+    v = v > kp1 ? v : kp1;
+    gzn[o] = v;
+    above = v;
+  }
+}
+
+// ----------------------------------------------------------------- updatedzd
+// cubic_spline_interpolation_from_layer_center_to_interfaces for up to four
+// fields, one column per lane over the full domain.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    spline_kernel(Dom d, const T* __restrict__ gk, const T* __restrict__ beta,
+                  const T* __restrict__ gamma, const T* q0, const T* q1, const T* q2,
+                  const T* q3, T* o0, T* o1, T* o2_, T* o3) {
+  const int ni = d.iec - d.isc + 7, nj = d.jec - d.jsc + 7;
+  const int n = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (n >= ni * nj) return;
+  const int i = d.isc - 3 + n % ni, j = d.jsc - 3 + n / ni, t = (int)blockIdx.y;
+  const int f = (int)blockIdx.z;
+  const T* qc = f == 0 ? q0 : f == 1 ? q1 : f == 2 ? q2 : q3;
+  T* qi = f == 0 ? o0 : f == 1 ? o1 : f == 2 ? o2_ : o3;
+  const int nk = d.nk;
+  const int64_t base = at3(d, i, j, 0, t), sk = d.sk;
+  T prev;
+  {
+    const T xt1 = T(2) * gk[0] * (gk[0] + T(1));
+    prev = (xt1 * qc[base] + qc[base + sk]) / beta[0];
+    qi[base] = prev;
+  }
+  for (int k = 1; k < nk - 1; ++k) {
+    prev = (T(3) * (qc[base + (k - 1) * sk] + gk[k] * qc[base + k * sk]) - prev) / beta[k];
+    qi[base + k * sk] = prev;
+  }
+  {
+    const int k = nk - 1;
+    const T g = gk[k - 1];
+    const T a_bot = T(1) + g * (g + T(1.5));
+    const T xt1 = T(2) * g * (g + T(1));
+    const T xt2 = g * (g + T(0.5)) - a_bot * gamma[k - 1];
+    prev = (xt1 * qc[base + (k - 1) * sk] + qc[base + (k - 2) * sk] - a_bot * prev) / xt2;
+    qi[base + k * sk] = prev;
+  }
+  for (int k = nk - 2; k >= 0; --k) {
+    const T v = qi[base + k * sk] - gamma[k] * prev;
+    qi[base + k * sk] = v;
+    prev = v;
+  }
+}
+
+// apply_height_fluxes (updatedzd.py:63-118), one column per lane on the
+// compute domain.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    apply_height_kernel(Dom d, const T* __restrict__ area, const T* __restrict__ zs,
+                        T* __restrict__ h, const T* __restrict__ fx, const T* __restrict__ fy,
+                        const T* __restrict__ xfx, const T* __restrict__ yfx,
+                        const T* __restrict__ gxd, const T* __restrict__ gyd, T* __restrict__ ws,
+                        T dt) {
+  const int ni = d.iec - d.isc + 1, nj = d.jec - d.jsc + 1;
+  const int n = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (n >= ni * nj) return;
+  const int i = d.isc + n % ni, j = d.jsc + n / ni, t = (int)blockIdx.y;
+  const int64_t o2 = at2(d, i, j, t);
+  const T a = area[o2];
+  const int nk = d.nk;
+  const int64_t sj = d.sj;
+  for (int k = 0; k < nk; ++k) {
+    const int64_t o = at3(d, i, j, k, t);
+    const T after = (a + xfx[o] - xfx[o + 1]) + (a + yfx[o] - yfx[o + sj]) - a;
+    const T adv = (h[o] * a + fx[o] - fx[o + 1] + fy[o] - fy[o + sj]) / after;
+    h[o] = adv + (gxd[o] - gxd[o + 1] + gyd[o] - gyd[o + sj]) / a;
+  }
+  T above = h[at3(d, i, j, nk - 1, t)];
+  ws[o2] = (zs[o2] - above) / dt;
+  for (int k = nk - 2; k >= 0; --k) {
+    const int64_t o = at3(d, i, j, k, t);
+    const T other = above + T(kDzMin);
+    T v = h[o];
+    v = v > other ? v : other;
+    h[o] = v;
+    above = v;
+  }
+}
+
+// ------------------------------------------------------------------- rings
+// pk3_halo (width 2, pk3 = pe^akap) and pe_halo (width 1, pe) column sums.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    ring_kernel(Dom d, int width, int mode, T ptop, T akap, const T* __restrict__ delp,
+                T* __restrict__ out) {
+  const int ni = d.iec - d.isc + 1 + 2 * width, nj = d.jec - d.jsc + 1 + 2 * width;
+  const int n = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (n >= ni * nj) return;
+  const int i = d.isc - width + n % ni, j = d.jsc - width + n / ni, t = (int)blockIdx.y;
+  if (i >= d.isc && i <= d.iec && j >= d.jsc && j <= d.jec) return;
+  T pe = ptop;
+  const int64_t base = at3(d, i, j, 0, t), sk = d.sk;
+  if (mode == 1) out[base] = ptop;
+  for (int k = 1; k < d.nk; ++k) {
+    pe = pe + delp[base + (k - 1) * sk];
+    out[base + k * sk] = mode == 0 ? pow(pe, akap) : pe;
+  }
+}
+
+// ------------------------------------------------------------------ ray_fast
+// RayleighDamping (ray_fast.py:25-127) with the per-level rf, masks and
+// column mass dm prepared on the host; one column per lane on [isc, iec+1]^2.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    ray_fast_kernel(Dom d, int nonhydro, const T* __restrict__ rf, const T* __restrict__ dp,
+                    const T* __restrict__ dm, const int* __restrict__ use_rf,
+                    const int* __restrict__ use_n, T* __restrict__ u, T* __restrict__ v,
+                    T* __restrict__ w) {
+  const int ni = d.iec - d.isc + 2, nj = d.jec - d.jsc + 2;
+  const int n = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (n >= ni * nj) return;
+  const int i = d.isc + n % ni, j = d.jsc + n / ni, t = (int)blockIdx.y;
+  const int64_t base = at3(d, i, j, 0, t), sk = d.sk;
+  const int nk = d.nk;
+  auto damp = [&](T* q) {
+    T dmdir = T(0);
+    for (int k = 0; k < nk; ++k) {
+      if (use_rf[k]) {
+        const int64_t o = base + k * sk;
+        dmdir = dmdir + (T(1) - rf[k]) * dp[k] * q[o];
+        q[o] = q[o] * rf[k];
+      }
+    }
+    for (int k = 0; k < nk; ++k) {
+      if (use_n[k]) {
+        const int64_t o = base + k * sk;
+        q[o] = q[o] + dmdir / dm[k];
+      }
+    }
+  };
+  if (i <= d.iec) damp(u);
+  if (j <= d.jec) damp(v);
+  if (nonhydro && i <= d.iec && j <= d.jec) {
+    for (int k = 0; k < nk; ++k)
+      if (use_rf[k]) w[base + k * sk] = w[base + k * sk] * rf[k];
+  }
+}
+
+}  // namespace
+
+int check_metrics_list(const fv3_metrics* g, const void* const* ptrs, int n, const char* who) {
+  FV3_REQUIRE(g != nullptr, FV3_E_ARG, "%s: NULL metrics", who);
+  for (int a = 0; a < n; ++a)
+    FV3_REQUIRE(ptrs[a] != nullptr, FV3_E_ARG, "%s: a metric term is NULL", who);
+  return FV3_OK;
+}
+
+}  // namespace fv3
+
+extern "C" {
+
+int32_t fv3_update_dz_c_f64(const fv3_domain* dom, const double* area, const double* zs,
+                            const double* dp_ref, const double* ut, const double* vt,
+                            double* gz, double* ws, double dt, double* work, int64_t work_slab,
+                            void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_update_dz_c";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(area && zs && dp_ref && ut && vt && gz && ws && work, FV3_E_ARG,
+              "%s: NULL pointer", who);
+  FV3_REQUIRE(dom->nk >= 3, FV3_E_ARG, "%s: needs at least 3 interface levels", who);
+  if (dom->ntiles == 0) return FV3_OK;
+  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)dom->nk * dom->sk,
+              FV3_E_ARG, "%s: work_slab too small", who);
+  Dom d = to_dom(dom);
+  hipStream_t s = as_stream(stream);
+  const int ncol = (d.iec - d.isc + 3) * (d.jec - d.jsc + 3);
+  hipLaunchKernelGGL((update_dz_c_kernel<double>), dim3((ncol + 255) / 256, d.ntiles), dim3(256),
+                     0, s, d, dp_ref, zs, area, ut, vt, gz, work, ws, 1.0 / dt);
+  FV3_HIP_CHECK_LAUNCH(who);
+  const Box bx{d.isc - 1, d.iec + 1, d.jsc - 1, d.jec + 1};
+  const double* src = work;
+  launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    gz[o] = src[o];
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+int32_t fv3_update_dz_d_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t hord_tm,
+                            const int32_t nord4[4], const double* damp, const double* gk,
+                            const double* beta, const double* gamma, const double* zs,
+                            double* height, const double* crx, const double* cry,
+                            const double* xfx, const double* yfx, double* ws, double dt,
+                            double* work, int64_t work_slab, void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_update_dz_d";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  const void* need[] = {g ? g->area : nullptr, g ? g->dxa : nullptr, g ? g->dya : nullptr,
+                        g ? g->rarea : nullptr, g ? g->del6_u : nullptr,
+                        g ? g->del6_v : nullptr};
+  rc = check_metrics_list(g, need, 6, who);
+  if (rc) return rc;
+  FV3_REQUIRE(nord4 && damp && gk && beta && gamma && zs && height && crx && cry && xfx && yfx &&
+                  ws && work,
+              FV3_E_ARG, "%s: NULL pointer", who);
+  FV3_REQUIRE(dom->nk >= 3, FV3_E_ARG, "%s: needs at least 3 interface levels", who);
+  if (dom->ntiles == 0) return FV3_OK;
+  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)dom->nk * dom->sk,
+              FV3_E_ARG, "%s: work_slab too small", who);
+  Dom d = to_dom(dom);
+  hipStream_t s = as_stream(stream);
+  double* W[9];
+  for (int n = 0; n < 9; ++n) W[n] = work + (int64_t)n * work_slab;
+  double *crx_i = W[0], *xfx_i = W[1], *cry_i = W[2], *yfx_i = W[3], *fx = W[4], *fy = W[5];
+  double *gxd = W[6], *gyd = W[7], *qy = W[8];
+  const int ncol = (d.iec - d.isc + 7) * (d.jec - d.jsc + 7);
+  hipLaunchKernelGGL((spline_kernel<double>), dim3((ncol + 255) / 256, d.ntiles, 4), dim3(256), 0,
+                     s, d, gk, beta, gamma, crx, xfx, cry, yfx, crx_i, xfx_i, cry_i, yfx_i);
+  FV3_HIP_CHECK_LAUNCH(who);
+  rc = fv3_copy_corners_xy_f64(dom, height, height, qy, stream);
+  if (rc) return rc;
+  const double* area = static_cast<const double*>(g->area);
+  rc = fv3_fvtp2d_f64(dom, hord_tm, height, qy, crx_i, cry_i, xfx_i, yfx_i, nullptr, nullptr,
+                      area, static_cast<const double*>(g->dxa),
+                      static_cast<const double*>(g->dya), fx, fy, stream);
+  if (rc) return rc;
+  rc = fv3_delnflux_f64(dom, FV3_DELN_STORE, nord4, height, nullptr, damp,
+                        static_cast<const double*>(g->del6_u), static_cast<const double*>(g->del6_v),
+                        static_cast<const double*>(g->rarea), gxd, gyd, stream);
+  if (rc) return rc;
+  const int nc = (d.iec - d.isc + 1) * (d.jec - d.jsc + 1);
+  hipLaunchKernelGGL((apply_height_kernel<double>), dim3((nc + 255) / 256, d.ntiles), dim3(256), 0,
+                     s, d, area, zs, height, fx, fy, xfx_i, yfx_i, gxd, gyd, ws, dt);
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+int32_t fv3_nh_p_grad_f64(const fv3_domain* dom, const fv3_metrics* g, double* u, double* v,
+                          double* pp, double* gz, double* pk3, const double* delp, double dt,
+                          double ptk, double* work, int64_t work_slab, void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_nh_p_grad";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  rc = check_a2b_metrics(g, who);
+  if (rc) return rc;
+  FV3_REQUIRE(g->rdx && g->rdy, FV3_E_ARG, "%s: rdx/rdy are NULL", who);
+  FV3_REQUIRE(u && v && pp && gz && pk3 && delp && work, FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  const int nz = dom->nk;
+  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)(nz + 1) * dom->sk,
+              FV3_E_ARG, "%s: work_slab too small", who);
+  hipStream_t s = as_stream(stream);
+  const Met<double> m = to_met<double>(g);
+  double *wk = work, *wk1 = work + work_slab, *qx = work + 2 * work_slab;
+  double *qy = work + 3 * work_slab, *edges = work + 4 * work_slab;
+  // a2b_k1 on pp and pk3 (levels 1..nz), a2b_kbuffer on gz (0..nz), replace
+  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pp, wk1, qx, qy, edges, 1, pp, who);
+  if (rc) return rc;
+  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pk3, wk1, qx, qy, edges, 1, pk3, who);
+  if (rc) return rc;
+  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz + 1, gz, wk1, qx, qy, edges, 1, gz, who);
+  if (rc) return rc;
+  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz, delp, wk1, qx, qy, edges, 0, nullptr, who);
+  if (rc) return rc;
+  Dom d = to_dom(dom);
+  // set_k0_and_calc_wk on [isc, iec+1]^2
+  const Box bc{d.isc, d.iec + 1, d.jsc, d.jec + 1};
+  launch_box(s, d, bc, 0, nz, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    if (k == 0) {
+      pp[o] = 0.0;
+      pk3[o] = ptk;
+    }
+    wk[o] = pk3[o + d.sk] - pk3[o];
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  // calc_u on [isc, iec] x [jsc, jec+1] and calc_v on [isc, iec+1] x [jsc, jec]
+  launch_box(s, d, bc, 0, nz, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const int64_t o2 = at2(d, i, j, t);
+    const int64_t sk = d.sk;
+    if (i <= d.iec) {
+      const int64_t q = o + 1;
+      const double du = dt / (wk[o] + wk[q]) *
+                        ((gz[o + sk] - gz[q]) * (pk3[q + sk] - pk3[o]) +
+                         (gz[o] - gz[q + sk]) * (pk3[o + sk] - pk3[q]));
+      u[o] = (u[o] + du +
+              dt / (wk1[o] + wk1[q]) *
+                  ((gz[o + sk] - gz[q]) * (pp[q + sk] - pp[o]) +
+                   (gz[o] - gz[q + sk]) * (pp[o + sk] - pp[q]))) *
+             m.rdx[o2];
+    }
+    if (j <= d.jec) {
+      const int64_t q = o + d.sj;
+      const double dv = dt / (wk[o] + wk[q]) *
+                        ((gz[o + sk] - gz[q]) * (pk3[q + sk] - pk3[o]) +
+                         (gz[o] - gz[q + sk]) * (pk3[o + sk] - pk3[q]));
+      v[o] = (v[o] + dv +
+              dt / (wk1[o] + wk1[q]) *
+                  ((gz[o + sk] - gz[q]) * (pp[q + sk] - pp[o]) +
+                   (gz[o] - gz[q + sk]) * (pp[o + sk] - pp[q]))) *
+             m.rdy[o2];
+    }
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+int32_t fv3_pk3_halo_f64(const fv3_domain* dom, double* pk3, const double* delp, double ptop,
+                         double akap, void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_pk3_halo";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(pk3 && delp, FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const int nc = (d.iec - d.isc + 5) * (d.jec - d.jsc + 5);
+  hipLaunchKernelGGL((ring_kernel<double>), dim3((nc + 255) / 256, d.ntiles), dim3(256), 0,
+                     as_stream(stream), d, 2, 0, ptop, akap, delp, pk3);
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+int32_t fv3_pe_halo_f64(const fv3_domain* dom, double* pe, const double* delp, double ptop,
+                        void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_pe_halo";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(pe && delp, FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const int nc = (d.iec - d.isc + 3) * (d.jec - d.jsc + 3);
+  hipLaunchKernelGGL((ring_kernel<double>), dim3((nc + 255) / 256, d.ntiles), dim3(256), 0,
+                     as_stream(stream), d, 1, 1, ptop, 0.0, delp, pe);
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+int32_t fv3_p_grad_c_f64(const fv3_domain* dom, const double* rdxc, const double* rdyc,
+                         int32_t hydrostatic, double* uc, double* vc, const double* delpc,
+                         const double* pkc, const double* gz, double dt2, void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_p_grad_c";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(rdxc && rdyc && uc && vc && pkc && gz && (hydrostatic || delpc), FV3_E_ARG,
+              "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const Box bx{d.isc, d.iec + 1, d.jsc, d.jec + 1};
+  launch_box(as_stream(stream), d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const int64_t o2 = at2(d, i, j, t);
+    const int64_t sk = d.sk, sj = d.sj;
+    auto wk = [&](int64_t q) { return hydrostatic ? pkc[q + sk] - pkc[q] : delpc[q]; };
+    const double w0 = wk(o);
+    {
+      const int64_t q = o - 1;
+      uc[o] = uc[o] + dt2 * rdxc[o2] / (wk(q) + w0) *
+                          ((gz[q + sk] - gz[o]) * (pkc[o + sk] - pkc[q]) +
+                           (gz[q] - gz[o + sk]) * (pkc[q + sk] - pkc[o]));
+    }
+    {
+      const int64_t q = o - sj;
+      vc[o] = vc[o] + dt2 * rdyc[o2] / (wk(q) + w0) *
+                          ((gz[q + sk] - gz[o]) * (pkc[o + sk] - pkc[q]) +
+                           (gz[q] - gz[o + sk]) * (pkc[q + sk] - pkc[o]));
+    }
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+int32_t fv3_ray_fast_f64(const fv3_domain* dom, int32_t hydrostatic, const double* rf,
+                         const double* dp, const double* dm, const int32_t* use_rf,
+                         const int32_t* use_nudge, double* u, double* v, double* w,
+                         void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_ray_fast";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(rf && dp && dm && use_rf && use_nudge && u && v && (hydrostatic || w), FV3_E_ARG,
+              "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const int nc = (d.iec - d.isc + 2) * (d.jec - d.jsc + 2);
+  hipLaunchKernelGGL((ray_fast_kernel<double>), dim3((nc + 255) / 256, d.ntiles), dim3(256), 0,
+                     as_stream(stream), d, hydrostatic ? 0 : 1, rf, dp, dm, use_rf, use_nudge, u,
+                     v, w);
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+int32_t fv3_del2cubed_f64(const fv3_domain* dom, const double* del6_u, const double* del6_v,
+                          const double* rarea, int32_t nmax, double cd, double* qdel,
+                          double* work, int64_t work_slab, void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_del2cubed";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(del6_u && del6_v && rarea && qdel && work, FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)dom->nk * dom->sk,
+              FV3_E_ARG, "%s: work_slab too small", who);
+  Dom d = to_dom(dom);
+  hipStream_t s = as_stream(stream);
+  double* q = work;
+  double* qy = work + work_slab;
+  const int ntimes = nmax < 3 ? nmax : 3;
+  const Box full{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
+  for (int n = 0; n < ntimes; ++n) {
+    const int nt = ntimes - (n + 1);
+    // corner_fill (del2cubed.py:30-62): q = qdel with three-cell averages
+    // at the 12 points around the tile corners
+    launch_box(s, d, full, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      const int64_t sj = d.sj;
+      const double third = 1.0 / 3.0;
+      double val = qdel[o];
+      const bool w = i == d.is || i == d.is - 1, e = i == d.ie || i == d.ie + 1;
+      const bool so = j == d.js || j == d.js - 1, no = j == d.je || j == d.je + 1;
+      if ((w || e) && (so || no)) {
+        const int ci = w ? d.is : d.ie, cj = so ? d.js : d.je;
+        const int ox = w ? -1 : 1, oy = so ? -1 : 1;  // outward directions
+        if (i == ci && j == cj) {
+          val = (qdel[o] + qdel[o + ox] + qdel[o + oy * sj]) * third;
+        } else if (i == ci + ox && j == cj) {
+          val = (qdel[o - ox] + qdel[o] + qdel[o - ox + oy * sj]) * third;
+        } else if (i == ci && j == cj + oy) {
+          val = (qdel[o - oy * sj] + qdel[o + ox - oy * sj] + qdel[o]) * third;
+        }
+      }
+      q[o] = val;
+    });
+    FV3_HIP_CHECK_LAUNCH(who);
+    const double* qxp = q;
+    const double* qyp = q;
+    if (nt > 0) {
+      rc = fv3_copy_corners_xy_f64(dom, q, q, qy, stream);
+      if (rc) return rc;
+      qyp = qy;
+    }
+    // copy to qdel and the flux update on the compute domain + nt halo
+    launch_box(s, d, full, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      double val = qyp[o];
+      if (i >= d.isc - nt && i <= d.iec + nt && j >= d.jsc - nt && j <= d.jec + nt) {
+        const int64_t o2 = at2(d, i, j, t);
+        const int64_t sj = d.sj, sj2 = d.sj2;
+        const double fx0 = del6_v[o2] * (qxp[o - 1] - qxp[o]);
+        const double fx1 = del6_v[o2 + 1] * (qxp[o] - qxp[o + 1]);
+        const double fy0 = del6_u[o2] * (qyp[o - sj] - qyp[o]);
+        const double fy1 = del6_u[o2 + sj2] * (qyp[o] - qyp[o + sj]);
+        val = val + cd * rarea[o2] * (fx0 - fx1 + fy0 - fy1);
+      }
+      qdel[o] = val;
+    });
+    FV3_HIP_CHECK_LAUNCH(who);
+  }
+  return FV3_OK;
+}
+
+int32_t fv3_pkz_tempadjust_f64(const fv3_domain* dom, const double* delp, const double* delz,
+                               const double* cappa, const double* heat_source, double* pt,
+                               double* pkz, double delt_time_factor, void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_pkz_tempadjust";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(delp && delz && cappa && heat_source && pt && pkz, FV3_E_ARG, "%s: NULL pointer",
+              who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const Box comp{d.isc, d.iec, d.jsc, d.jec};
+  const double rdg = -287.05 / 9.80665;
+  const double cv_air = 1004.6 - 287.05;
+  launch_box(as_stream(stream), d, comp, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const double c = cappa[o];
+    const double pk = pow(rdg * delp[o] / delz[o] * pt[o], c / (1.0 - c));
+    pkz[o] = pk;
+    const double dtmp = heat_source[o] / (cv_air * delp[o]);
+    const double f = k == 0 ? delt_time_factor * 0.1
+                            : (k == 1 ? delt_time_factor * 0.5 : delt_time_factor);
+    const double deltmin = sgn(fmin(f, fabs(dtmp)), dtmp);
+    pt[o] = pt[o] + deltmin / pk;
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+int32_t fv3_dyncore_column_f64(const fv3_domain* dom, int32_t op, double scalar, const double* a,
+                               const double* b, double* out, void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_dyncore_column";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(out && (op == 4 || a), FV3_E_ARG, "%s: NULL pointer", who);
+  FV3_REQUIRE(op >= 0 && op <= 4, FV3_E_ARG, "%s: unknown op %d", who, op);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  hipStream_t s = as_stream(stream);
+  if (op == 0) {
+    // set_gz (dyn_core.py:80-85): out = gz on nk levels, a = zs (IJ), b = delz
+    FV3_REQUIRE(b, FV3_E_ARG, "%s: NULL delz", who);
+    const Box comp{d.isc, d.iec, d.jsc, d.jec};
+    launch_box(s, d, comp, 0, 1, [=] __device__(int i, int j, int, int t) {
+      int64_t o = at3(d, i, j, d.nk - 1, t);
+      double g = a[at2(d, i, j, t)];
+      out[o] = g;
+      for (int k = d.nk - 2; k >= 0; --k) {
+        o -= d.sk;
+        g = g - b[o];
+        out[o] = g;
+      }
+    });
+  } else if (op == 1) {
+    // set_pem (dyn_core.py:88-93): out = pem, a = delp, scalar = ptop
+    const Box bx{d.isc - 1, d.iec + 1, d.jsc - 1, d.jec + 1};
+    launch_box(s, d, bx, 0, 1, [=] __device__(int i, int j, int, int t) {
+      int64_t o = at3(d, i, j, 0, t);
+      double p = scalar;
+      out[o] = p;
+      for (int k = 1; k < d.nk; ++k) {
+        o += d.sk;
+        p = p + a[o];
+        out[o] = p;
+      }
+    });
+  } else if (op == 2) {
+    // compute_geopotential (dyn_core.py:96-98) on compute + 2 halo: out = a * GRAV
+    const Box bx{d.isc - 2, d.iec + 2, d.jsc - 2, d.jec + 2};
+    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      out[o] = a[o] * 9.80665;
+    });
+  } else if (op == 3) {
+    // copy_defn on the full domain
+    const Box bx{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
+    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      out[o] = a[o];
+    });
+  } else {
+    // fill the full domain with a value
+    const Box bx{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
+    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      out[at3(d, i, j, k, t)] = scalar;
+    });
+  }
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+}

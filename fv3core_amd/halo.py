@@ -152,6 +152,42 @@ def build_map(n: int, staggers: Sequence[Tuple[float, float]], n_halo: int = 3, 
     return out[order]
 
 
+def build_sync_map(n: int, h: int = 3) -> np.ndarray:
+    """Index map of synchronize_vector_interfaces for a D-grid pair (u, v).
+
+    On a cube edge the two tiles both compute the wind component lying on
+    that edge line: u on a north/south edge, v on an east/west edge.  The
+    tile for which the edge is its south or west edge keeps its value and
+    the neighbour, for which the same line is its north or east edge, takes
+    it over, rotated where the tiles' axes rotate (the Fortran dyn_core
+    fills u(:, je+1) and v(ie+1, :) from the neighbours' buffers the same
+    way).  Rows as in :func:`build_map`.
+    """
+    staggers = (Y_FACE, X_FACE)
+    rows: List[Tuple[int, ...]] = []
+    lo, hi = float(h), float(h + n)
+    for t in range(6):
+        for edge, comp in (("north", 0), ("east", 1)):
+            tb, edge_b, rot = neighbour(t, edge)
+            jac = _jacobian(n, h, edge, edge_b, rot)
+            col = jac[:, comp]
+            src_comp = int(np.argmax(np.abs(col)))
+            sign = int(np.sign(col[src_comp]))
+            stg, stg_b = staggers[comp], staggers[src_comp]
+            for m in range(n):
+                sv = 0.5 + m
+                X, Y = (lo + sv, hi) if edge == "north" else (hi, lo + sv)
+                Xb, Yb = _into_neighbour(n, h, edge_b, rot, 0.0, sv)
+                i, j = X - stg[0], Y - stg[1]
+                ib, jb = Xb - stg_b[0], Yb - stg_b[1]
+                assert abs(ib - round(ib)) < 1e-9 and abs(jb - round(jb)) < 1e-9
+                rows.append((t, int(round(i)), int(round(j)), comp, tb, int(round(ib)),
+                             int(round(jb)), src_comp * 2 + (1 if sign < 0 else 0)))
+    out = np.array(rows, dtype=np.int32)
+    order = np.lexsort((out[:, 1], out[:, 2], out[:, 3], out[:, 0]))
+    return out[order]
+
+
 def apply_map_numpy(fields: Sequence[np.ndarray], mp: np.ndarray) -> List[np.ndarray]:
     """Host restatement of the gather, used by the tests as the checker."""
     out = [f.copy() for f in fields]
@@ -185,6 +221,7 @@ class HaloUpdater:
         "vector_d": (Y_FACE, X_FACE),
         "vector_c": (X_FACE, Y_FACE),
         "corner": (CORNER,),
+        "sync_d": (Y_FACE, X_FACE),
     }
     _cache: Dict[Tuple, torch.Tensor] = {}
 
@@ -202,7 +239,10 @@ class HaloUpdater:
     def _device_map(self, device) -> torch.Tensor:
         key = (self._n, self._kind, self._n_halo, str(device))
         if key not in self._cache:
-            mp = build_map(self._n, self._KINDS[self._kind], self._n_halo, self._gi.n_halo)
+            if self._kind == "sync_d":
+                mp = build_sync_map(self._n, self._gi.n_halo)
+            else:
+                mp = build_map(self._n, self._KINDS[self._kind], self._n_halo, self._gi.n_halo)
             self._cache[key] = torch.as_tensor(mp, device=device)
         return self._cache[key]
 
@@ -251,6 +291,14 @@ class CubedSphereCommunicator:
     def get_vector_halo_updater(self, specs_x=None, specs_y=None, kind: str = "vector_d",
                                 n_halo: int = 3):
         return _GroupUpdater(HaloUpdater(self._gi, kind, n_halo))
+
+    def synchronize_vector_interfaces(self, x_quantity, y_quantity):
+        """Make the cube-edge values of a D-grid wind pair agree between tiles
+        (see :func:`build_sync_map`)."""
+        if not hasattr(self, "_sync"):
+            self._sync = HaloUpdater(self._gi, "sync_d")
+        fields = _GroupUpdater._storages([x_quantity, y_quantity])
+        self._sync.update(*fields)
 
 
 class _GroupUpdater:

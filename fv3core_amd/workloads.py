@@ -260,3 +260,91 @@ def riemann_inputs(n: int = 24, nz: int = 79, workload: int = 2, n_halo: int = 3
     return dict(ak=ak, bk=bk, ptop=ptop, delp=delp, q_con=q_con, cappa=cappa, pt=pt,
                 delz=delz, zh=zh, zs=zs, hs=zs * G, w=w, ws=np.zeros((ni, nj)), gz=gz,
                 pe=pe.copy() if pe.shape[2] == nk else pe, dt=dt)
+
+
+def synthetic_hybrid(nz: int, ptop: float = 300.0, p0: float = 1.0e5):
+    """(ak, bk) of a smooth hybrid grid for any nz: pure pressure at the top,
+    terrain-following at the bottom (bk = eta^3, ak = p_ref - bk p0 >= 0)."""
+    eta = np.arange(nz + 1) / nz
+    p_ref = ptop + (p0 - ptop) * eta ** 1.5
+    bk = eta ** 3
+    ak = p_ref - bk * p0
+    return ak, bk, ptop
+
+
+def cube_state(n: int = 96, nz: int = 79, seed: int = 0, ps: float = 1.0e5):
+    """The DynCore initial state on the six cube tiles (arrays [6, I, J, K]).
+
+    Hybrid ak/bk levels (the 79-level grid for nz = 79), ps = 1e5 Pa,
+    T = 288 K with a 6.5 K/km lapse rate down to a 216 K tropopause
+    (expressed through pressure), a zonal jet u = 35 sin^2(2 lat) m/s
+    projected on the D-grid edges, a seeded 1 K Gaussian pt bump, w = 0,
+    dry air (cappa = Rd/cp, q_con = 0), flat surface.  pt is T / p**cappa,
+    delz the hydrostatic thickness, pe/peln/pk/pkz consistent with delp.
+    """
+    from . import cube
+    from .constants import CP_AIR, GRAV, KAPPA, RDGAS
+
+    rng = np.random.default_rng(seed)
+    if nz == 79:
+        ak, bk, ptop = hybrid_coefficients(79)
+    else:
+        ak, bk, ptop = synthetic_hybrid(nz)
+    m = cube.tile_metrics(n)
+    h = 3
+    ni = n + 2 * h + 1
+    nk = nz + 1
+    shp = (6, ni, ni, nk)
+    pe = np.zeros(shp)
+    pe[...] = ak[None, None, None, :] + bk[None, None, None, :] * ps
+    delp = np.zeros(shp)
+    delp[..., :nz] = np.diff(pe, axis=3)
+    delp[..., nz] = delp[..., nz - 1]
+    peln = np.log(pe)
+    pk = np.exp(KAPPA * peln)
+    pm = np.ones(shp)
+    pm[..., :nz] = delp[..., :nz] / (peln[..., 1:] - peln[..., :nz])
+    gamma = 0.0065 * RDGAS / GRAV
+    temp = np.maximum(216.0, 288.0 * (pm / ps) ** gamma)
+    cappa = np.full(shp, RDGAS / CP_AIR)
+    pt = temp * np.exp(-cappa * np.log(pm))
+    # seeded Gaussian bump of 1 K centred at a random point of tile 0..5
+    P = m["agrid3"]
+    c = rng.normal(size=3)
+    c /= np.linalg.norm(c)
+    dist = np.arccos(np.clip(np.einsum("tijx,x->tij", P, c), -1, 1))
+    bump = np.exp(-(dist / 0.2) ** 2)[..., None]
+    pt = pt * (1.0 + bump / temp)
+    delz = np.zeros(shp)
+    delz[..., :nz] = -(delp[..., :nz] / GRAV) * RDGAS * pt[..., :nz] * \
+        np.exp((cappa[..., :nz] - 1.0) * np.log(pm[..., :nz]))
+    zh = np.zeros(shp)
+    for k in range(nz - 1, -1, -1):
+        zh[..., k] = zh[..., k + 1] - delz[..., k]
+    pkz = np.ones(shp)
+    pkz[..., :nz] = (pk[..., 1:] - pk[..., :nz]) / (KAPPA * (peln[..., 1:] - peln[..., :nz]))
+    # zonal jet on the D-grid: u on y-faces (edge P(i,j) -> P(i+1,j)), v on x-faces
+    G = m["grid3"]
+
+    def edge_wind(p1, p2):
+        mid = p1 + p2
+        mid = mid / np.linalg.norm(mid, axis=-1, keepdims=True)
+        tang = p2 - p1
+        tang = tang / np.linalg.norm(tang, axis=-1, keepdims=True)
+        lat = np.arcsin(np.clip(mid[..., 2], -1, 1))
+        east = np.stack([-mid[..., 1], mid[..., 0], np.zeros_like(mid[..., 0])], axis=-1)
+        east = east / np.maximum(np.linalg.norm(east, axis=-1, keepdims=True), 1e-12)
+        U = 35.0 * np.sin(2.0 * lat) ** 2
+        return U * np.einsum("...x,...x->...", east, tang)
+
+    u = np.zeros(shp)
+    v = np.zeros(shp)
+    u[:, :ni - 1, :, :] = edge_wind(G[:, :-1, :], G[:, 1:, :])[..., None]
+    v[:, :, :ni - 1, :] = edge_wind(G[:, :, :-1], G[:, :, 1:])[..., None]
+    z = np.zeros(shp)
+    state = dict(delp=delp, pt=pt, u=u, v=v, w=z.copy(), delz=delz, q_con=z.copy(),
+                 cappa=cappa, uc=z.copy(), vc=z.copy(), ua=z.copy(), va=z.copy(),
+                 omga=z.copy(), mfxd=z.copy(), mfyd=z.copy(), cxd=z.copy(), cyd=z.copy(),
+                 diss_estd=z.copy(), pe=pe, pk=pk, peln=peln, pkz=pkz, zh=zh)
+    return dict(state=state, ak=ak, bk=bk, ptop=ptop, metric=m,
+                phis=np.zeros((6, ni, ni)), wsd=np.zeros((6, ni, ni)))

@@ -306,6 +306,88 @@ int32_t fv3_d_sw_f64(const fv3_domain* d, const fv3_metrics* g, const fv3_dsw_pa
                      const fv3_dsw_state* st, double dt, double* work, int64_t work_slab,
                      void* stream);
 
+/*
+ * UpdateGeopotentialHeightOnCGrid.__call__ (fv3core/stencils/updatedzc.py:122-163):
+ * gz (d->nk interface levels) advected by the C-grid winds ut, vt on
+ * [isc-1, iec+1]^2 with the DZ_MIN floor; ws (IJ) = (zs - gz_surface) / dt.
+ * dp_ref: device column of layer thicknesses.  work: 1 slab.
+ */
+int32_t fv3_update_dz_c_f64(const fv3_domain* d, const double* area, const double* zs,
+                            const double* dp_ref, const double* ut, const double* vt,
+                            double* gz, double* ws, double dt, double* work, int64_t work_slab,
+                            void* stream);
+
+/*
+ * UpdateHeightOnDGrid.__call__ (fv3core/stencils/updatedzd.py:323-393) on
+ * d->nk interface levels: spline interpolation of crx, xfx, cry, yfx to the
+ * interfaces (device columns gk, beta, gamma from the reference's
+ * cubic_spline_interpolation_constants), fv_tp_2d of height (hord_tm),
+ * DelnFluxNoSG damping (nord4, per-level damp), the flux update with the
+ * DZ_MIN floor and ws.  work: 9 slabs.
+ */
+int32_t fv3_update_dz_d_f64(const fv3_domain* d, const fv3_metrics* g, int32_t hord_tm,
+                            const int32_t nord4[4], const double* damp, const double* gk,
+                            const double* beta, const double* gamma, const double* zs,
+                            double* height, const double* crx, const double* cry,
+                            const double* xfx, const double* yfx, double* ws, double dt,
+                            double* work, int64_t work_slab, void* stream);
+
+/*
+ * NonHydrostaticPressureGradient.__call__ (fv3core/stencils/nh_p_grad.py:147-206)
+ * on d->nk layers: pp, pk3 (levels 1..nk) and gz (0..nk) replaced by their
+ * B-grid values, then u, v updated (and multiplied by rdx, rdy).
+ * ptk = ptop ** akap.  work: 5 slabs of nk + 1 levels.
+ */
+int32_t fv3_nh_p_grad_f64(const fv3_domain* d, const fv3_metrics* g, double* u, double* v,
+                          double* pp, double* gz, double* pk3, const double* delp, double dt,
+                          double ptk, double* work, int64_t work_slab, void* stream);
+
+/* PK3Halo.__call__ (fv3core/stencils/pk3_halo.py:11-68): pk3 = pe ** akap on
+ * the 2-wide ring around the compute domain, levels 1..d->nk-1. */
+int32_t fv3_pk3_halo_f64(const fv3_domain* d, double* pk3, const double* delp, double ptop,
+                         double akap, void* stream);
+
+/* edge_pe (fv3core/stencils/pe_halo.py:6-33): pe on the 1-wide ring. */
+int32_t fv3_pe_halo_f64(const fv3_domain* d, double* pe, const double* delp, double ptop,
+                        void* stream);
+
+/* p_grad_c_stencil (fv3core/stencils/dyn_core.py:101-145) on [isc, iec+1]^2. */
+int32_t fv3_p_grad_c_f64(const fv3_domain* d, const double* rdxc, const double* rdyc,
+                         int32_t hydrostatic, double* uc, double* vc, const double* delpc,
+                         const double* pkc, const double* gz, double dt2, void* stream);
+
+/*
+ * RayleighDamping.__call__ (fv3core/stencils/ray_fast.py:130-192) with the
+ * per-level factors prepared on the host: rf, dp, dm (column mass of the
+ * nudged levels) and the masks pfull < rf_cutoff, pfull < rf_cutoff_nudge.
+ */
+int32_t fv3_ray_fast_f64(const fv3_domain* d, int32_t hydrostatic, const double* rf,
+                         const double* dp, const double* dm, const int32_t* use_rf,
+                         const int32_t* use_nudge, double* u, double* v, double* w,
+                         void* stream);
+
+/* HyperdiffusionDamping.__call__ (fv3core/stencils/del2cubed.py:158-181):
+ * min(3, nmax) del-2 passes on qdel with coefficient cd.  work: 2 slabs. */
+int32_t fv3_del2cubed_f64(const fv3_domain* d, const double* del6_u, const double* del6_v,
+                          const double* rarea, int32_t nmax, double cd, double* qdel,
+                          double* work, int64_t work_slab, void* stream);
+
+/* compute_pkz_tempadjust (fv3core/stencils/temperature_adjust.py:8-44) on
+ * the compute domain, levels [0, d->nk). */
+int32_t fv3_pkz_tempadjust_f64(const fv3_domain* d, const double* delp, const double* delz,
+                               const double* cappa, const double* heat_source, double* pt,
+                               double* pkz, double delt_time_factor, void* stream);
+
+/*
+ * Small stencils of dyn_core (fv3core/stencils/dyn_core.py:47-98) by op:
+ * 0 set_gz (out = gz, a = zs IJ, b = delz), 1 set_pem (out = pem, a = delp,
+ * scalar = ptop), 2 compute_geopotential (out = a * GRAV on compute + 2),
+ * 3 copy a -> out on the full domain, 4 fill out with scalar on the full
+ * domain; d->nk levels.
+ */
+int32_t fv3_dyncore_column_f64(const fv3_domain* d, int32_t op, double scalar, const double* a,
+                               const double* b, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,165 @@
+"""AcousticDynamics (dyn_core) on the six-tile cube: GPU against the oracle
+driver, which runs the oracle's c_sw, d_sw, Riemann solvers, height
+updates, pressure gradients, Rayleigh damping and del2cubed tile by tile
+with the host restatement of the halo map between them."""
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from fv3core_amd._config import DynamicalCoreConfig
+from fv3core_amd.halo import CENTER, CORNER, X_FACE, Y_FACE, apply_map_numpy, build_map, \
+    build_sync_map
+from fv3core_amd.workloads import cube_state
+from oracle import dsw as ods
+from oracle import dyncore as odc
+from tests.helpers import assert_close
+
+N, NZ = 12, 8
+
+
+def init_pfull(ak, bk, p_ref=1.0e5):
+    """init_pfull (fv_dynamics.py:42-51)."""
+    ph1 = ak[:-1] + bk[:-1] * p_ref
+    ph2 = ak[1:] + bk[1:] * p_ref
+    return (ph2 - ph1) / np.log(ph2 / ph1)
+
+
+def _config(n_split=2):
+    return DynamicalCoreConfig(npz=NZ, n_split=n_split, k_split=1, nord=3, dt_atmos=225)
+
+
+def _temporaries(shape):
+    tmp = {k: np.zeros(shape) for k in ("ut", "vt", "gz", "zh_tmp", "pem", "pkc", "pk3",
+                                         "heat_source", "divgd", "crx", "xfx", "cry", "yfx",
+                                         "delpc", "ptc")}
+    tmp["gz"][:] = 1.0e40
+    tmp["pk3"][:] = 1.0e40
+    return tmp
+
+
+class _Halo:
+    def __init__(self, state):
+        self.state = state
+        self.maps = {}
+
+    def _map(self, kind, n_halo):
+        key = (kind, n_halo)
+        if key not in self.maps:
+            stg = {"scalar": (CENTER,), "vector_d": (Y_FACE, X_FACE),
+                   "vector_c": (X_FACE, Y_FACE), "corner": (CORNER,)}[kind]
+            self.maps[key] = build_map(N, stg, n_halo, 3)
+        return self.maps[key]
+
+    def __call__(self, kind, names, n_halo=3):
+        mp = self._map(kind, n_halo)
+        if kind.startswith("vector"):
+            for a in range(0, len(names), 2):
+                u, v = apply_map_numpy([self.state[names[a]], self.state[names[a + 1]]], mp)
+                self.state[names[a]][...] = u
+                self.state[names[a + 1]][...] = v
+        else:
+            for name in names:
+                self.state[name][...] = apply_map_numpy([self.state[name]], mp)[0]
+
+    def sync(self):
+        mp = build_sync_map(N, 3)
+        u, v = apply_map_numpy([self.state["u"], self.state["v"]], mp)
+        self.state["u"][...] = u
+        self.state["v"][...] = v
+
+
+def test_sync_map_joins_the_edge_winds():
+    """After synchronize_vector_interfaces the u on every tile's north edge
+    and v on its east edge equal the neighbour's (rotated) values, and the
+    south/west edges are left alone."""
+    rng = np.random.default_rng(3)
+    shape = (6, N + 7, N + 7, 2)
+    u, v = rng.normal(size=shape), rng.normal(size=shape)
+    mp = build_sync_map(N, 3)
+    u2, v2 = apply_map_numpy([u, v], mp)
+    h = 3
+    np.testing.assert_array_equal(u2[:, :, h], u[:, :, h])  # south rows untouched
+    np.testing.assert_array_equal(v2[:, h, :], v[:, h, :])  # west columns untouched
+    changed = (u2 != u).any(axis=-1)
+    assert changed[:, h:h + N, h + N].all() and changed.sum() == 6 * N
+    # applying it twice changes nothing more
+    u3, v3 = apply_map_numpy([u2, v2], mp)
+    np.testing.assert_array_equal(u3, u2)
+    np.testing.assert_array_equal(v3, v2)
+
+
+@pytest.mark.gpu
+def test_gpu_acoustic_dynamics_matches_oracle():
+    from fv3core_amd.halo import CubedSphereCommunicator
+    from fv3core_amd.stencils.d_sw import get_column_namelist
+    from fv3core_amd.stencils.dyn_core import AcousticDynamics
+    from fv3core_amd.utils import gt4py_utils as utils
+    from fv3core_amd.utils.grid import DampingCoefficients, GridData
+    from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
+
+    init = cube_state(N, NZ, seed=2)
+    st0 = init["state"]
+    m = init["metric"]
+    ak, bk, ptop = init["ak"], init["bk"], init["ptop"]
+    pfull = init_pfull(ak, bk)
+    cfg = _config()
+    gi = GridIndexing((N, N, NZ), 3, True, True, True, True)
+    factory = StencilFactory(StencilConfig(), gi)
+
+    def dev4(a):
+        t = utils.empty(a.shape)
+        t.copy_(torch.as_tensor(a))
+        return t
+
+    def dev3(a):
+        t = utils.empty(a.shape, batch_dims=1)
+        t.copy_(torch.as_tensor(a))
+        return t
+
+    grid = GridData({k: dev3(v) for k, v in m.items()
+                     if isinstance(v, np.ndarray) and v.ndim == 3})
+    damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
+    comm = CubedSphereCommunicator(gi)
+    acoustic = AcousticDynamics(comm, factory, grid, damp, 0, False, False, cfg.acoustic_dynamics,
+                                ak, bk, pfull, dev3(init["phis"]))
+    state = types.SimpleNamespace(**{k: dev4(v) for k, v in st0.items()})
+    state.phis = dev3(init["phis"])
+    state.wsd = dev3(init["wsd"])
+    state.ptop, state.ks, state.mdt, state.n_map = ptop, 0, float(cfg.dt_atmos), 1
+    acoustic(state)
+    torch.cuda.synchronize()
+
+    # oracle
+    S = {k: v.copy() for k, v in st0.items()}
+    S["phis"] = init["phis"].copy()
+    S["wsd"] = init["wsd"].copy()
+    S["ws3"] = np.zeros_like(init["wsd"])
+    tmp = _temporaries(st0["delp"].shape)
+    tmp.pop("zh_tmp")
+    S.update(tmp)
+    halo = _Halo(S)
+    dsw = cfg.d_grid_shallow_water
+    col = get_column_namelist(dsw, NZ)
+    ocfg = dict(mdt=float(cfg.dt_atmos), n_split=cfg.n_split, k_split=cfg.k_split, n_map=1,
+                ptop=ptop, hord_dp=cfg.hord_dp, hord_tm=cfg.hord_tm, hord_mt=cfg.hord_mt,
+                hord_vt=cfg.hord_vt, dddmp=cfg.dddmp, d4_bg=cfg.d4_bg, nord=cfg.nord,
+                d_con=cfg.d_con, convert_ke=cfg.convert_ke, vtdm4=cfg.vtdm4, p_fac=cfg.p_fac,
+                rf_cutoff=cfg.rf_cutoff, tau=cfg.tau, delt_max=cfg.delt_max)
+    metrics = [{k: v[t] for k, v in m.items() if isinstance(v, np.ndarray) and v.ndim == 3}
+               for t in range(6)]
+    odc.acoustic_dynamics(S, ocfg, metrics, dict(da_min=m["da_min"], da_min_c=m["da_min_c"]),
+                          ak, bk, pfull, halo, halo.sync, col, NZ)
+    c = (slice(None), slice(3, N + 3), slice(3, N + 3), slice(0, NZ))
+    for name in ("delp", "pt", "w", "delz", "q_con", "pkz"):
+        got = getattr(state, name).cpu().numpy()
+        assert_close(got[c], S[name][c], 1e-9, near_zero=1e-12, what=name, scale_floor=1e-6)
+    cu = (slice(None), slice(3, N + 3), slice(3, N + 4), slice(0, NZ))
+    cv = (slice(None), slice(3, N + 4), slice(3, N + 3), slice(0, NZ))
+    assert_close(state.u.cpu().numpy()[cu], S["u"][cu], 1e-9, what="u", scale_floor=1e-6)
+    assert_close(state.v.cpu().numpy()[cv], S["v"][cv], 1e-9, what="v", scale_floor=1e-6)
+    ci = (slice(None), slice(3, N + 3), slice(3, N + 3), slice(0, NZ + 1))
+    for name in ("pe", "peln", "pk", "zh"):
+        got = getattr(state, name).cpu().numpy()
+        assert_close(got[ci], S[name][ci], 1e-9, what=name, scale_floor=1e-6)

@@ -1,0 +1,318 @@
+"""Acoustic-step stencils besides c_sw/d_sw: updatedzc, updatedzd, nh_p_grad,
+pk3/pe halos, p_grad_c, ray_fast, del2cubed, temperature adjustment.
+Oracle pins (CPU) and GPU parity."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dyncore as odc
+from oracle.fvtp2d import Tile
+from tests.dyn_helpers import Case
+from tests.helpers import assert_close
+
+NZ = 6
+
+
+def _dp0(nz):
+    return np.linspace(300.0, 1800.0, nz)
+
+
+def _zh_profile(case, nz):
+    """Interface heights decreasing with k plus horizontal structure."""
+    base = np.linspace(12000.0, 0.0, nz + 1)
+    out = []
+    for a in case.field(0.0, 150.0):
+        out.append(base[None, None, :] + a[..., :nz + 1] * np.linspace(1, 0, nz + 1))
+    return out
+
+
+# ------------------------------------------------------------ oracle pins (CPU)
+def test_oracle_spline_reproduces_linear_profiles():
+    dp0 = np.full(NZ + 1, 500.0)
+    gk, beta, gamma = odc.spline_constants(dp0, NZ + 1)
+    kk = np.arange(NZ + 1)[None, None, :] * np.ones((3, 3, 1))
+    qi = odc.spline_to_interfaces(2.0 - 0.3 * kk, gk, beta, gamma)
+    np.testing.assert_allclose(qi, 2.0 - 0.3 * (kk - 0.5), rtol=1e-14)
+
+
+def test_oracle_update_dz_keeps_level_surfaces_flat():
+    """A height field that is uniform on each level stays so under any
+    winds (advective form with the matching area fluxes) in both the C- and
+    D-grid updates; ws is then (zs - surface height) / dt."""
+    case = Case(12, NZ, cube_tiles=False, seed=5)
+    t, g = case.tile, case.metric[0]
+    s = case.shape
+    base = np.linspace(9000.0, 0.0, NZ + 1)
+    gz = np.broadcast_to(base, s).copy()
+    ut, vt = case.field(0.0, 3.0e7)[0], case.field(0.0, 3.0e7)[0]
+    ws = np.zeros(s[:2])
+    zs = np.full(s[:2], 10.0)
+    odc.update_dz_c(_dp0(NZ + 1), zs, g["area"], ut, vt, gz, ws, 15.0, t)
+    c = (slice(t.isc, t.iec + 1), slice(t.jsc, t.jec + 1))
+    np.testing.assert_allclose(gz[c], np.broadcast_to(base, gz[c].shape), rtol=1e-12)
+    np.testing.assert_allclose(ws[c], (10.0 - 0.0) / 15.0, atol=1e-9)
+    h = np.broadcast_to(base, s).copy()
+    crx, cry = case.field(0.0, 0.2)[0], case.field(0.0, 0.2)[0]
+    xfx, yfx = crx * g["dy"][..., None] * 1e5, cry * g["dx"][..., None] * 1e5
+    ws2 = np.zeros(s[:2])
+    col = np.full(NZ + 1, 0.06)
+    odc.update_dz_d(zs, h, crx, cry, xfx, yfx, ws2, 30.0, g, t, 6, [0, 0, 2, 2], col,
+                    _dp0(NZ + 1))
+    np.testing.assert_allclose(h[c], np.broadcast_to(base, h[c].shape), rtol=1e-12)
+
+
+def test_oracle_dz_min_floor():
+    """Layers thinner than DZ_MIN are opened up to it from the surface."""
+    case = Case(12, NZ, cube_tiles=False, seed=6)
+    t, g = case.tile, case.metric[0]
+    s = case.shape
+    base = np.array([30.0, 29.5, 29.0, 10.0, 9.9, 1.0, 0.0])
+    gz = np.broadcast_to(base, s).copy()
+    z = np.zeros(s)
+    ws = np.zeros(s[:2])
+    odc.update_dz_c(_dp0(NZ + 1), np.zeros(s[:2]), g["area"], z, z, gz, ws, 10.0, t)
+    c = (t.isc + 2, t.jsc + 2)
+    # each interface ends at least DZ_MIN above the one below it
+    np.testing.assert_allclose(gz[c], [33.0, 31.0, 29.0, 11.9, 9.9, 2.0, 0.0])
+
+
+def test_oracle_nh_p_grad_flat_surfaces_exert_no_force():
+    """With pressures and heights uniform on every level there is no
+    horizontal gradient: u, v only change units (times rdx, rdy)."""
+    case = Case(12, NZ, cube_tiles=False, seed=7)
+    t, g = case.tile, case.metric[0]
+    s = case.shape
+    pp = np.broadcast_to(np.linspace(0, 50.0, NZ + 1), s).copy()
+    pk3 = np.broadcast_to(np.linspace(5.0, 25.0, NZ + 1), s).copy()
+    gz = np.broadcast_to(np.linspace(9e4, 0.0, NZ + 1), s).copy()
+    delp = np.full(s, 800.0)
+    u0, v0 = case.field(20.0, 10.0)[0] * 1e5, case.field(-5.0, 10.0)[0] * 1e5
+    u, v = u0.copy(), v0.copy()
+    odc.nh_p_grad(u, v, pp, gz, pk3, delp[..., :NZ], 30.0, 300.0, odc.KAPPA, g, t)
+    cu = (slice(t.isc, t.iec + 1), slice(t.jsc, t.jec + 2), slice(0, NZ))
+    np.testing.assert_allclose(u[cu], u0[cu] / 1e5, rtol=1e-12)
+    cv = (slice(t.isc, t.iec + 2), slice(t.jsc, t.jec + 1), slice(0, NZ))
+    np.testing.assert_allclose(v[cv], v0[cv] / 1e5, rtol=1e-12)
+
+
+def test_oracle_ray_fast_conserves_column_momentum():
+    """Rayleigh damping moves the removed momentum back into the nudged
+    layers: sum_k dp_k u_k over the column is unchanged."""
+    case = Case(12, NZ, cube_tiles=False, seed=8)
+    t = case.tile
+    pfull = np.array([50.0, 400.0, 1500.0, 2900.0, 3050.0, 50000.0])
+    dp = np.array([90.0, 500.0, 1200.0, 1500.0, 150.0, 20000.0])
+    u0, v0, w0 = (case.field(10.0, 20.0)[0] for _ in range(3))
+    u, v, w = u0.copy(), v0.copy(), w0.copy()
+    odc.ray_fast(u[..., :NZ], v[..., :NZ], w[..., :NZ], dp, pfull, 150.0, 30.0, 3000.0, 10.0, t)
+    c = (slice(t.isc, t.iec + 1), slice(t.jsc, t.jec + 1))
+    m0 = (u0[c][..., :NZ] * dp).sum(axis=-1)
+    m1 = (u[c][..., :NZ] * dp).sum(axis=-1)
+    np.testing.assert_allclose(m1, m0, rtol=1e-13)
+    assert np.abs(u[c][..., 0] - u0[c][..., 0]).max() > 1e-3  # the top was damped
+    np.testing.assert_array_equal(u[c][..., 5], u0[c][..., 5])  # below the nudging
+
+
+def test_oracle_del2cubed_conserves_on_periodic_tile():
+    case = Case(12, NZ, cube_tiles=False, seed=9)
+    t, g = case.tile, case.metric[0]
+    q = case.field(1.0, 0.5)[0][..., :NZ].copy()
+    q0 = q.copy()
+    odc.del2cubed(q, 0.2 * case.scalars["da_min"], g, t, 3)
+    c = (slice(t.isc, t.iec + 1), slice(t.jsc, t.jec + 1))
+    area = g["area"][c][..., None]
+    np.testing.assert_allclose((q[c] * area).sum((0, 1)), (q0[c] * area).sum((0, 1)), rtol=1e-13)
+    assert np.abs(q[c] - q0[c]).max() > 1e-4
+
+
+def test_oracle_pe_halo_is_the_column_sum():
+    case = Case(12, NZ, cube_tiles=False, seed=10)
+    t = case.tile
+    delp = case.field(900.0, 100.0)[0]
+    pe = np.zeros(case.shape)
+    odc.pe_halo(pe, delp, 300.0, t)
+    i, j = t.isc - 1, t.jsc + 4
+    np.testing.assert_allclose(pe[i, j], 300.0 + np.concatenate([[0.0], np.cumsum(delp[i, j, :NZ])]))
+    pk3 = np.zeros(case.shape)
+    odc.pk3_halo(pk3, delp, 300.0, odc.KAPPA, t)
+    i, j = t.iec + 2, t.jsc
+    np.testing.assert_allclose(pk3[i, j, 1:], (300.0 + np.cumsum(delp[i, j, :NZ])) ** odc.KAPPA,
+                               rtol=1e-14)
+    assert pk3[t.isc, t.jsc, 1] == 0.0  # the compute domain is left alone
+
+
+# ------------------------------------------------------------- GPU parity
+def _dev(case, arrs):
+    return case.dev(arrs)
+
+
+def _dev2(case, arrs):
+    import fv3core_amd.utils.gt4py_utils as utils
+
+    if case.ntiles == 1:
+        t = utils.empty(arrs[0].shape)
+        t.copy_(torch.as_tensor(arrs[0]))
+        return t
+    t = utils.empty((6,) + arrs[0].shape, batch_dims=1)
+    t.copy_(torch.as_tensor(np.stack(arrs)))
+    return t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cube_tiles", [False, True])
+def test_gpu_update_dz_c_and_d(cube_tiles):
+    from fv3core_amd.stencils.updatedzc import UpdateGeopotentialHeightOnCGrid
+    from fv3core_amd.stencils.updatedzd import UpdateHeightOnDGrid
+    from fv3core_amd.utils.grid import DampingCoefficients
+
+    case = Case(16, NZ, cube_tiles, seed=51)
+    grid = case.grid_data()
+    zh = _zh_profile(case, NZ)
+    ut, vt = case.field(0.0, 2e7), case.field(0.0, 2e7)
+    zs = [np.full(case.shape[:2], 30.0) for _ in range(case.ntiles)]
+    ws = [np.zeros(case.shape[:2]) for _ in range(case.ntiles)]
+    dp0 = _dp0(NZ + 1)
+    gz_d = _dev(case, zh)
+    ws_d = _dev2(case, ws)
+    udc = UpdateGeopotentialHeightOnCGrid(case.factory, grid.area)
+    udc(dp0, _dev2(case, zs), _dev(case, ut), _dev(case, vt), gz_d, ws_d, 18.75)
+    crx, cry = case.field(0.0, 0.15), case.field(0.0, 0.15)
+    xfx = [c * m["dy"][..., None] * 2e4 for c, m in zip(crx, case.metric)]
+    yfx = [c * m["dx"][..., None] * 2e4 for c, m in zip(cry, case.metric)]
+    col = {"damp_vt": np.full(NZ + 1, 0.06), "nord_v": np.array([0, 0, 2] + [2] * (NZ - 2))}
+    damp = DampingCoefficients(grid.del6_u, grid.del6_v, case.scalars["da_min"],
+                               case.scalars["da_min_c"])
+    udd = UpdateHeightOnDGrid(case.factory, damp, grid, 0, 6, dp0, col, [[0, 1], [1, 1], [2, 1], [3, 1]])
+    h_d = _dev(case, zh)
+    ws2_d = _dev2(case, ws)
+    udd(_dev2(case, zs), h_d, _dev(case, crx), _dev(case, cry), _dev(case, xfx), _dev(case, yfx),
+        ws2_d, 37.5)
+    torch.cuda.synchronize()
+    t = case.tile
+    for ti in range(case.ntiles):
+        g = case.metric[ti]
+        gz = zh[ti].copy()
+        wsr = np.zeros(case.shape[:2])
+        odc.update_dz_c(dp0, zs[ti], g["area"], ut[ti], vt[ti], gz, wsr, 18.75, t)
+        box = (slice(t.isc - 1, t.iec + 2), slice(t.jsc - 1, t.jec + 2))
+        assert_close(case.host(gz_d)[ti][box], gz[box], 1e-13, what=f"tile {ti} gz (c)")
+        wsg = ws_d.cpu().numpy() if case.ntiles == 1 else ws_d.cpu().numpy()[ti]
+        assert_close(wsg[box], wsr[box], 1e-12, near_zero=1e-12, what=f"tile {ti} ws (c)")
+        h = zh[ti].copy()
+        ws2r = np.zeros(case.shape[:2])
+        odc.update_dz_d(zs[ti], h, crx[ti], cry[ti], xfx[ti], yfx[ti], ws2r, 37.5, g, t, 6,
+                        col["nord_v"], col["damp_vt"], dp0)
+        c = (slice(t.isc, t.iec + 1), slice(t.jsc, t.jec + 1))
+        assert_close(case.host(h_d)[ti][c], h[c], 1e-12, what=f"tile {ti} height (d)")
+        ws2g = ws2_d.cpu().numpy() if case.ntiles == 1 else ws2_d.cpu().numpy()[ti]
+        assert_close(ws2g[c], ws2r[c], 1e-10, near_zero=1e-10, what=f"tile {ti} ws (d)")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cube_tiles", [False, True])
+def test_gpu_nh_p_grad_and_halos(cube_tiles):
+    from fv3core_amd.stencils.nh_p_grad import NonHydrostaticPressureGradient
+    from fv3core_amd.stencils.pk3_halo import PK3Halo, edge_pe
+
+    case = Case(16, NZ, cube_tiles, seed=61)
+    grid = case.grid_data()
+    pp = case.field(0.0, 30.0)
+    pk3 = [a + np.linspace(5.0, 25.0, NZ + 1) for a in case.field(0.0, 0.3)]
+    gz = [a * 9.80665 for a in _zh_profile(case, NZ)]
+    delp = case.field(800.0, 60.0)
+    u, v = case.field(20.0 * 1e5, 8e5), case.field(-3.0e5, 8e5)
+    d = {k: _dev(case, a) for k, a in dict(pp=pp, pk3=pk3, gz=gz, delp=delp, u=u, v=v).items()}
+    nh = NonHydrostaticPressureGradient(case.factory, grid, 0)
+    nh(d["u"], d["v"], d["pp"], d["gz"], d["pk3"], d["delp"], 37.5, 300.0, odc.KAPPA)
+    pkh = _dev(case, pk3)
+    PK3Halo(case.factory)(pkh, d["delp"], 300.0, odc.KAPPA)
+    pe0 = case.field(0.0, 1.0)
+    ped = _dev(case, pe0)
+    edge_pe(case.factory.grid_indexing, ped, d["delp"], 300.0)
+    torch.cuda.synchronize()
+    t = case.tile
+    full = (slice(0, case.n + 6), slice(0, case.n + 6))
+    for ti in range(case.ntiles):
+        g = case.metric[ti]
+        r = {k: a[ti].copy() for k, a in dict(pp=pp, pk3=pk3, gz=gz, delp=delp, u=u, v=v).items()}
+        odc.nh_p_grad(r["u"], r["v"], r["pp"], r["gz"], r["pk3"], r["delp"][..., :NZ], 37.5,
+                      300.0, odc.KAPPA, g, t)
+        for name in ("u", "v"):
+            assert_close(case.host(d[name])[ti][full][..., :NZ], r[name][full][..., :NZ], 1e-11,
+                         what=f"tile {ti} {name}")
+        for name in ("pp", "pk3", "gz"):
+            assert_close(case.host(d[name])[ti][full], r[name][full], 1e-12, near_zero=1e-300,
+                         what=f"tile {ti} {name}")
+        pk = pk3[ti].copy()
+        odc.pk3_halo(pk, delp[ti], 300.0, odc.KAPPA, t)
+        assert_close(case.host(pkh)[ti][full], pk[full], 1e-14, what=f"tile {ti} pk3 halo")
+        pe = pe0[ti].copy()
+        odc.pe_halo(pe, delp[ti], 300.0, t)
+        assert_close(case.host(ped)[ti][full], pe[full], 0.0, what=f"tile {ti} pe halo")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cube_tiles", [False, True])
+def test_gpu_pgrad_c_rayleigh_del2cubed_tempadjust(cube_tiles):
+    import ctypes
+
+    from fv3core_amd import _lib
+    from fv3core_amd.stencils.del2cubed import HyperdiffusionDamping
+    from fv3core_amd.stencils.ray_fast import RayleighDamping
+    from fv3core_amd.stencils.temperature_adjust import compute_pkz_tempadjust
+    from fv3core_amd.utils.grid import DampingCoefficients
+
+    case = Case(16, NZ, cube_tiles, seed=71)
+    grid = case.grid_data()
+    gi = case.factory.grid_indexing
+    uc, vc = case.field(10.0, 10.0), case.field(-2.0, 10.0)
+    delpc = case.field(900.0, 50.0)
+    pkc = [a + np.linspace(0.0, 40.0, NZ + 1) for a in case.field(0.0, 2.0)]
+    gz = [a * 9.80665 for a in _zh_profile(case, NZ)]
+    d = {k: _dev(case, a) for k, a in dict(uc=uc, vc=vc, delpc=delpc, pkc=pkc, gz=gz).items()}
+    dom = _lib.make_domain(gi, d["uc"], NZ, grid.rdxc)
+    _lib.check(_lib.lib().fv3_p_grad_c_f64(
+        ctypes.byref(dom), _lib.ptr(grid.rdxc), _lib.ptr(grid.rdyc), 0, _lib.ptr(d["uc"]),
+        _lib.ptr(d["vc"]), _lib.ptr(d["delpc"]), _lib.ptr(d["pkc"]), _lib.ptr(d["gz"]),
+        ctypes.c_double(18.75), ctypes.c_void_p(_lib.stream_handle())))
+    pfull = np.array([50.0, 400.0, 1500.0, 2900.0, 3050.0, 50000.0])
+    dp = np.array([90.0, 500.0, 1200.0, 1500.0, 150.0, 20000.0])
+    u, v, w = case.field(10.0, 20.0), case.field(-4.0, 20.0), case.field(0.0, 1.0)
+    du, dv, dw = _dev(case, u), _dev(case, v), _dev(case, w)
+    RayleighDamping(case.factory, 3000.0, 10.0, False)(du, dv, dw, dp, pfull, 150.0, 30.0, 0)
+    q = case.field(1.0, 0.5)
+    dq = _dev(case, q)
+    damp = DampingCoefficients(grid.del6_u, grid.del6_v, case.scalars["da_min"],
+                               case.scalars["da_min_c"])
+    cd = 0.2 * case.scalars["da_min"]
+    HyperdiffusionDamping(case.factory, damp, grid.rarea, 3)(dq, cd)
+    delp = case.field(900.0, 50.0)
+    delz = case.field(-250.0, 20.0)
+    cappa = case.field(0.2857, 0.001)
+    hs = case.field(0.0, 3e5)
+    pt = case.field(300.0, 10.0)
+    pkz = case.field(0.0, 1.0)
+    dd = {k: _dev(case, a) for k, a in dict(delp=delp, delz=delz, cappa=cappa, hs=hs, pt=pt,
+                                             pkz=pkz).items()}
+    compute_pkz_tempadjust(gi, NZ, dd["delp"], dd["delz"], dd["cappa"], dd["hs"], dd["pt"],
+                           dd["pkz"], 0.08)
+    torch.cuda.synchronize()
+    t = case.tile
+    full = (slice(0, case.n + 6), slice(0, case.n + 6))
+    for ti in range(case.ntiles):
+        g = case.metric[ti]
+        ru, rv = uc[ti].copy(), vc[ti].copy()
+        odc.p_grad_c(ru, rv, delpc[ti][..., :NZ], pkc[ti], gz[ti], 18.75, g, t)
+        assert_close(case.host(d["uc"])[ti][full], ru[full], 1e-13, what=f"tile {ti} uc")
+        assert_close(case.host(d["vc"])[ti][full], rv[full], 1e-13, what=f"tile {ti} vc")
+        a, b, c = u[ti].copy(), v[ti].copy(), w[ti].copy()
+        odc.ray_fast(a[..., :NZ], b[..., :NZ], c[..., :NZ], dp, pfull, 150.0, 30.0, 3000.0, 10.0, t)
+        for name, got, want in (("u", du, a), ("v", dv, b), ("w", dw, c)):
+            assert_close(case.host(got)[ti][full], want[full], 1e-14, what=f"tile {ti} {name}")
+        qq = q[ti][..., :NZ].copy()
+        odc.del2cubed(qq, cd, g, t, 3)
+        assert_close(case.host(dq)[ti][full][..., :NZ], qq[full], 1e-13, what=f"tile {ti} q")
+        rp, rk = pt[ti].copy(), pkz[ti].copy()
+        odc.pkz_tempadjust(delp[ti], delz[ti], cappa[ti], hs[ti], rp, rk, 0.08, t, NZ)
+        assert_close(case.host(dd["pt"])[ti][full], rp[full], 1e-13, what=f"tile {ti} pt")
+        assert_close(case.host(dd["pkz"])[ti][full], rk[full], 1e-13, what=f"tile {ti} pkz")
