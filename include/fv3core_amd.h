@@ -388,6 +388,57 @@ int32_t fv3_pkz_tempadjust_f64(const fv3_domain* d, const double* delp, const do
 int32_t fv3_dyncore_column_f64(const fv3_domain* d, int32_t op, double scalar, const double* a,
                                const double* b, double* out, void* stream);
 
+/* ---- vertical remapping (Lagrangian -> Eulerian) ---------------------- */
+
+/* Most tracers a remapping call handles (fv3core/utils/gt4py_utils.py:26-36). */
+#define FV3_MAX_TRACERS 9
+
+/*
+ * FillNegativeTracerValues.__call__ / fix_tracer (fv3core/stencils/
+ * fillz.py:12-150) on the compute box, levels [0, d->nk): borrows mass
+ * from the layers above and below a negative layer, then rescales the
+ * column below the top so its total mass is kept.  q: ntracers fields
+ * (<= FV3_MAX_TRACERS), dp: layer thicknesses.
+ */
+int32_t fv3_fillz_f64(const fv3_domain* d, int32_t ntracers, double* const* q, const double* dp,
+                      void* stream);
+
+/* Scalars of LagrangianToEulerian (fv3core/_config.py RemappingConfig and
+ * the call arguments of remapping.py:416-452). */
+typedef struct fv3_remap_params {
+  int32_t kord_tm, kord_tr, kord_wz, kord_mt; /* kord_tm < 0 */
+  int32_t nq;                                 /* tracers remapped (<= FV3_MAX_TRACERS) */
+  int32_t fill;                               /* fillz after the tracer remap */
+  int32_t last_step;                          /* moist_pt_last_step instead of pt / pkz */
+  int32_t pad_;
+  double ptop, akap, zvir;
+} fv3_remap_params;
+
+/* Fields of LagrangianToEulerian.__call__ (remapping.py:416-452), in the
+ * reference's tracer order qvapor, qliquid, qrain, qice, qsnow, qgraupel,
+ * qo3mr, qsgs_tke, qcld.  The first six tracers are always read (moist
+ * thermodynamics); the others only when n < nq.  ps and wsd are IJ fields;
+ * ak and bk hold d->nk + 1 values. */
+typedef struct fv3_remap_state {
+  double* tracers[FV3_MAX_TRACERS];
+  double *pt, *delp, *delz, *peln, *u, *v, *w, *ua, *cappa, *q_con, *pkz, *pk, *pe;
+  double* ps;
+  const double *wsd, *ak, *bk;
+} fv3_remap_state;
+
+/*
+ * LagrangianToEulerian.__call__ (fv3core/stencils/remapping.py:416-628)
+ * for kord_tm < 0, non-hydrostatic, nwat = 6, no saturation adjustment:
+ * remaps pt (log-pressure, T >= 184 K), the nq tracers (then fillz), w,
+ * delz, u and v from the Lagrangian interfaces pe to the hybrid levels
+ * ak + bk * ps, updates delp, peln, pk, pkz, cappa, q_con, ps, pe, ua.
+ * d->nk = number of layers (>= 6); every 3-D field holds d->nk + 1 levels.
+ * work: 15 slabs of work_slab elements, each laid out like the fields.
+ */
+int32_t fv3_lagrangian_to_eulerian_f64(const fv3_domain* d, const fv3_remap_params* p,
+                                       const fv3_remap_state* s, double* work,
+                                       int64_t work_slab, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
