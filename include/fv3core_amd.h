@@ -439,6 +439,24 @@ int32_t fv3_lagrangian_to_eulerian_f64(const fv3_domain* d, const fv3_remap_para
                                        const fv3_remap_state* s, double* work,
                                        int64_t work_slab, void* stream);
 
+/* ---- full dynamics step (fv_dynamics) --------------------------------- */
+
+/*
+ * compute_preamble (fv3core/stencils/fv_dynamics.py:54-106): fv_setup
+ * (moist_cv.py:197-228, moist_phys, nwat = 6) then pt_adjust
+ * (fv_dynamics.py:32-34) on the compute domain, levels [0, d->nk).
+ * q6: qvapor, qliquid, qrain, qice, qsnow, qgraupel.  Writes cvm, q_con,
+ * dp1, cappa, pkz and pt.
+ */
+int32_t fv3_fv_setup_f64(const fv3_domain* d, const double* const* q6, const double* delp,
+                         const double* delz, double* pt, double* q_con, double* cvm, double* pkz,
+                         double* cappa, double* dp1, void* stream);
+
+/* set_omega (fv3core/stencils/fv_dynamics.py:37-39): omga = delp / delz * w
+ * on the compute domain, levels [0, d->nk). */
+int32_t fv3_set_omega_f64(const fv3_domain* d, const double* delp, const double* delz,
+                          const double* w, double* omga, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

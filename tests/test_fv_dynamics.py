@@ -1,0 +1,192 @@
+"""DynamicalCore.step_dynamics on the six-tile cube: the preamble oracle
+pinned on the CPU, and the whole step on the GPU against the oracle driver
+(acoustic substeps, tracer advection, remapping, omega and its damping)."""
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from fv3core_amd._config import DynamicalCoreConfig
+from fv3core_amd.constants import KAPPA, RDGAS, ZVIR
+from fv3core_amd.halo import apply_map_numpy, build_map, CENTER
+from fv3core_amd.workloads import cube_state
+from oracle import fv_dynamics as ofd
+from oracle import remapping as orm
+from oracle.fvtp2d import Tile
+from tests.helpers import assert_close
+from tests.test_dyncore import _Halo, _temporaries
+
+N, NZ = 12, 8
+TRACERS = orm.TRACERS
+
+
+def moist_cube_state(n, nz, seed):
+    """cube_state plus the nine tracers: vapour decreasing upwards, small
+    condensates, and a qliquid that changes sign so that fillz acts."""
+    init = cube_state(n, nz, seed=seed)
+    st = init["state"]
+    rng = np.random.default_rng(seed + 100)
+    shp = st["delp"].shape
+    lev = np.linspace(0.2, 1.0, shp[-1])[None, None, None, :]
+    ii = np.arange(shp[1])[None, :, None, None] / shp[1]
+    jj = np.arange(shp[2])[None, None, :, None] / shp[2]
+    tt = np.arange(6)[:, None, None, None]
+
+    def wave(amp, mean):
+        p = rng.uniform(0, 2 * np.pi, size=3)
+        return mean + amp * np.sin(2 * np.pi * ii + p[0] + tt) * np.cos(2 * np.pi * jj + p[1]) \
+            * np.cos(np.pi * lev + p[2])
+
+    tr = {"qvapor": 8.0e-3 * lev ** 2 + wave(1.0e-3, 0.0) * lev}
+    for name in TRACERS[1:]:
+        tr[name] = np.abs(wave(1.0e-5, 2.0e-5))
+    tr["qliquid"] = wave(2.0e-6, 0.0)
+    tr["qcld"] = np.clip(wave(0.3, 0.3), 0.0, 1.0)
+    for k, v in tr.items():
+        tr[k] = np.ascontiguousarray(np.broadcast_to(v, shp))
+    init["tracers"] = tr
+    return init
+
+
+def _config(k_split=1, n_split=2, kord=9):
+    return DynamicalCoreConfig(npz=NZ, n_split=n_split, k_split=k_split, nord=3, dt_atmos=225,
+                               kord_tm=-kord, kord_tr=kord, kord_wz=kord, kord_mt=kord)
+
+
+# ------------------------------------------------------------ oracle pins (CPU)
+def test_oracle_fv_setup_dry_closed_form():
+    """Dry air: cappa = kappa, dp1 = q_con = 0, pkz = (rdg delp pt / delz)**kappa
+    and pt_adjust divides pt by pkz."""
+    t = Tile(4, 4, 3, True, True, True, True)
+    shp = (6, 11, 11, 3)
+    rng = np.random.default_rng(1)
+    S = {k: np.zeros(shp) for k in ("cvm", "q_con", "dp1", "cappa", "pkz") + TRACERS}
+    S["delp"] = rng.uniform(500, 1500, shp)
+    S["delz"] = -rng.uniform(50, 500, shp)
+    S["pt"] = rng.uniform(250, 320, shp)
+    pt0 = S["pt"].copy()
+    ofd.fv_setup_pt_adjust(S, t, 2)
+    c = (slice(None), slice(3, 7), slice(3, 7), slice(0, 2))
+    np.testing.assert_allclose(S["cappa"][c], KAPPA, rtol=1e-15)
+    assert not S["dp1"][c].any() and not S["q_con"][c].any()
+    pkz = (-RDGAS / 9.80665 * S["delp"][c] * pt0[c] / S["delz"][c]) ** KAPPA
+    np.testing.assert_allclose(S["pkz"][c], pkz, rtol=1e-13)
+    np.testing.assert_allclose(S["pt"][c], pt0[c] / pkz, rtol=1e-13)
+    # outside the compute domain nothing changes
+    np.testing.assert_array_equal(S["pt"][:, :3], pt0[:, :3])
+
+
+def test_oracle_fv_setup_moist_virtual_temperature():
+    """With vapour only, pt_adjust multiplies by (1 + zvir qv) and cappa drops
+    below kappa (cv of vapour exceeds that of dry air)."""
+    t = Tile(4, 4, 3, True, True, True, True)
+    shp = (6, 11, 11, 3)
+    S = {k: np.zeros(shp) for k in ("cvm", "q_con", "dp1", "cappa", "pkz") + TRACERS}
+    S["qvapor"][:] = 0.01
+    S["delp"] = np.full(shp, 1000.0)
+    S["delz"] = np.full(shp, -100.0)
+    S["pt"] = np.full(shp, 300.0)
+    ofd.fv_setup_pt_adjust(S, t, 2)
+    c = (slice(None), slice(3, 7), slice(3, 7), slice(0, 2))
+    np.testing.assert_allclose(S["dp1"][c], ZVIR * 0.01, rtol=1e-15)
+    assert (S["cappa"][c] < KAPPA).all()
+    np.testing.assert_allclose(S["pt"][c] * S["pkz"][c], 300.0 * (1.0 + ZVIR * 0.01),
+                               rtol=1e-13)
+
+
+# ------------------------------------------------------------------ GPU parity
+def _tracer_halo_fn():
+    mp = build_map(N, (CENTER,), 3, 3)
+    return lambda qs: [apply_map_numpy([q], mp)[0] for q in qs]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k_split,kord", [(1, 9), (2, 10)])
+def test_gpu_dynamical_core_step_matches_oracle(k_split, kord):
+    from fv3core_amd.halo import CubedSphereCommunicator
+    from fv3core_amd.stencils.d_sw import get_column_namelist
+    from fv3core_amd.stencils.fv_dynamics import DynamicalCore
+    from fv3core_amd.utils import gt4py_utils as utils
+    from fv3core_amd.utils.grid import DampingCoefficients, GridData
+    from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
+
+    init = moist_cube_state(N, NZ, seed=4)
+    st0 = init["state"]
+    tr0 = init["tracers"]
+    m = init["metric"]
+    ak, bk, ptop = init["ak"], init["bk"], init["ptop"]
+    cfg = _config(k_split=k_split, kord=kord)
+    gi = GridIndexing((N, N, NZ), 3, True, True, True, True)
+    factory = StencilFactory(StencilConfig(), gi)
+
+    def dev4(a):
+        t = utils.empty(a.shape)
+        t.copy_(torch.as_tensor(a))
+        return t
+
+    def dev3(a):
+        t = utils.empty(a.shape, batch_dims=1)
+        t.copy_(torch.as_tensor(a))
+        return t
+
+    grid = GridData({k: dev3(v) for k, v in m.items()
+                     if isinstance(v, np.ndarray) and v.ndim == 3})
+    damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
+    comm = CubedSphereCommunicator(gi)
+    phis = dev3(init["phis"])
+    core = DynamicalCore(comm, grid, factory, damp, cfg, ak, bk, phis)
+    state = {k: dev4(v) for k, v in st0.items() if k not in ("cappa",)}
+    state.update({k: dev4(v) for k, v in tr0.items()})
+    state["phis"] = phis
+    state["ps"] = dev3(st0["pe"][..., NZ].copy())
+    bdt = float(cfg.dt_atmos)
+    core.step_dynamics(state, 0.0, False, bdt, ptop, cfg.n_split, 0)
+    torch.cuda.synchronize()
+
+    # oracle
+    S = {k: v.copy() for k, v in st0.items()}
+    S.update({k: v.copy() for k, v in tr0.items()})
+    S["phis"] = init["phis"].copy()
+    S["ps"] = st0["pe"][..., NZ].copy()
+    S["wsd"] = np.zeros_like(init["wsd"])
+    S["ws3"] = np.zeros_like(init["wsd"])
+    S["te0_2d"] = np.zeros_like(init["wsd"])
+    for k in ("cappa", "dp1", "cvm"):
+        S[k] = np.zeros_like(st0["delp"])
+    tmp = _temporaries(st0["delp"].shape)
+    tmp["zh"] = tmp.pop("zh_tmp")
+    S.update(tmp)
+    halo = _Halo(S)
+    dsw = cfg.d_grid_shallow_water
+    col = get_column_namelist(dsw, NZ)
+    ocfg = dict(bdt=bdt, n_split=cfg.n_split, k_split=cfg.k_split, ptop=ptop,
+                hord_dp=cfg.hord_dp, hord_tm=cfg.hord_tm, hord_mt=cfg.hord_mt,
+                hord_vt=cfg.hord_vt, hord_tr=cfg.hord_tr, dddmp=cfg.dddmp, d4_bg=cfg.d4_bg,
+                nord=cfg.nord, d_con=cfg.d_con, convert_ke=cfg.convert_ke, vtdm4=cfg.vtdm4,
+                p_fac=cfg.p_fac, rf_cutoff=cfg.rf_cutoff, tau=cfg.tau, delt_max=cfg.delt_max,
+                kord_tm=cfg.kord_tm, kord_tr=cfg.kord_tr, kord_wz=cfg.kord_wz,
+                kord_mt=cfg.kord_mt, fill=cfg.fill, z_tracer=cfg.z_tracer,
+                nf_omega=cfg.nf_omega, p_ref=cfg.p_ref)
+    metrics = [{k: v[ti] for k, v in m.items() if isinstance(v, np.ndarray) and v.ndim == 3}
+               for ti in range(6)]
+    ofd.fv_dynamics(S, ocfg, metrics, dict(da_min=m["da_min"], da_min_c=m["da_min_c"]), ak,
+                    bk, halo, _tracer_halo_fn(), halo.sync, col, NZ)
+
+    c = (slice(None), slice(3, N + 3), slice(3, N + 3), slice(0, NZ))
+    for name in ("delp", "pt", "w", "delz", "q_con", "pkz", "omga") + TRACERS[:8]:
+        got = state[name].cpu().numpy()
+        assert_close(got[c], S[name][c], 1e-9, near_zero=1e-14, what=name, scale_floor=1e-6)
+    cu = (slice(None), slice(3, N + 3), slice(3, N + 4), slice(0, NZ))
+    cv = (slice(None), slice(3, N + 4), slice(3, N + 3), slice(0, NZ))
+    assert_close(state["u"].cpu().numpy()[cu], S["u"][cu], 1e-9, what="u", scale_floor=1e-6)
+    assert_close(state["v"].cpu().numpy()[cv], S["v"][cv], 1e-9, what="v", scale_floor=1e-6)
+    ci = (slice(None), slice(3, N + 3), slice(3, N + 3), slice(0, NZ + 1))
+    for name in ("pe", "peln", "pk"):
+        assert_close(state[name].cpu().numpy()[ci], S[name][ci], 1e-9, what=name,
+                     scale_floor=1e-6)
+    assert_close(state["ps"].cpu().numpy()[:, 3:N + 3, 3:N + 3], S["ps"][:, 3:N + 3, 3:N + 3],
+                 1e-12, what="ps")
+    for name in ("mfxd", "mfyd", "cxd", "cyd"):
+        assert_close(state[name].cpu().numpy()[c], S[name][c], 1e-9, what=name,
+                     scale_floor=1e-6)
