@@ -3,17 +3,24 @@
 Prints one JSON line.  A step is one pass of the hot path over one batch of
 synthetic input with every input already resident in HBM.
 
-Workload "fvtp2d": FiniteVolumeTransport (fv_tp_2d, hord=8, area fluxes) on
-6 tiles of 96x96x79 (the C96 cube shape; each tile doubly periodic, inputs
-built like the single-tile transport workload), FP64.  One step is one
-call = one fused kernel launch.  Cell-level updates per step =
-6 * 96 * 96 * 79.
+Workload "dyncore" (default): AcousticDynamics (dyn_core: c_sw, d_sw,
+riem_solver_c, riem_solver3, updatedzc/updatedzd, nh_p_grad, Rayleigh
+damping, the on-device six-tile halo updates) on six C96 tiles 96x96x79 in
+FP64, n_split = 6 acoustic substeps per call, dt = 225 s, from the
+hydrostatic jet state of ``workloads.cube_state`` (hybrid levels, ps = 1e5,
+T = 288 K lapsing to 216 K, u = 35 sin^2(2 lat), seeded 1 K pt bump, w = 0).
+One step is one call; cell-level updates per step = 6 * 96 * 96 * 79 * 6
+(every cell advanced through every acoustic substep).
 
-The roofline object prices the fused kernel with its algorithmic bytes
-(q, crx, cry, x_area_flux, y_area_flux read, q_x_flux, q_y_flux written:
-7 fields x 8 B = 56 B per cell) over its average launch time measured with
-HIP events on the launch stream.  The CPU baseline times the numpy oracle on
-one 48x48x79 tile (single core).
+Workload "fvtp2d": FiniteVolumeTransport (fv_tp_2d, hord=8, area fluxes) on
+6 tiles of 96x96x79, FP64, one fused kernel per step; cell-level updates per
+step = 6 * 96 * 96 * 79.
+
+The roofline object prices one kernel of the step (the fused fv_tp_2d
+kernel unless --roofline-kernel names another) with its algorithmic bytes
+over its average launch time, measured with HIP events recorded on the
+kernel's own stream by the library during the timed region.  The CPU
+baseline times the numpy oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -22,6 +29,7 @@ import json
 import os
 import sys
 import time
+import types
 
 import numpy as np
 
@@ -35,9 +43,12 @@ HBM_PEAK_GBPS = 8000.0
 def _parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=200)
-    p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--workload", default="fvtp2d", choices=["fvtp2d"])
+    p.add_argument("--steps", type=int, default=None)
+    p.add_argument("--warmup", type=int, default=None)
+    p.add_argument("--workload", default="dyncore", choices=["dyncore", "fvtp2d"])
+    p.add_argument("--roofline-kernel", default="fvtp2d_kernel",
+                   choices=["fvtp2d_kernel", "riem3_kernel", "riem_c_kernel",
+                            "map_single_kernel"])
     p.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -73,6 +84,149 @@ def cpu_baseline(seconds: float):
         "kind": "port",
         "sample": f"numpy oracle fv_tp_2d hord=8, one 48x48x79 periodic tile, median of "
         f"{len(times)} calls ({med * 1e3:.1f} ms/call), host {os.cpu_count()} cpus visible",
+    }
+
+
+def cpu_baseline_dyncore(seconds: float):
+    """Oracle AcousticDynamics (numpy, FP64, one core) on six C24 tiles x 79
+    levels, n_split = 2: a bounded sample of the C96 workload."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import warnings
+
+    from fv3core_amd._config import DynamicalCoreConfig
+    from fv3core_amd.stencils.d_sw import get_column_namelist
+    from fv3core_amd.workloads import cube_state
+    from oracle import dyncore as odc
+    from oracle.fv_dynamics import init_pfull
+    from oracle.host_halo import HostHalo, acoustic_state
+
+    n, nz, n_split = 24, 79, 2
+    init = cube_state(n, nz, seed=0)
+    st0, m = init["state"], init["metric"]
+    ak, bk, ptop = init["ak"], init["bk"], init["ptop"]
+    cfg = DynamicalCoreConfig(npz=nz, n_split=n_split, k_split=1, nord=3, dt_atmos=225)
+    metrics = [{k: v[t] for k, v in m.items() if isinstance(v, np.ndarray) and v.ndim == 3}
+               for t in range(6)]
+    col = get_column_namelist(cfg.d_grid_shallow_water, nz)
+    ocfg = dict(mdt=float(cfg.dt_atmos), n_split=n_split, k_split=1, n_map=1, ptop=ptop,
+                hord_dp=cfg.hord_dp, hord_tm=cfg.hord_tm, hord_mt=cfg.hord_mt,
+                hord_vt=cfg.hord_vt, dddmp=cfg.dddmp, d4_bg=cfg.d4_bg, nord=cfg.nord,
+                d_con=cfg.d_con, convert_ke=cfg.convert_ke, vtdm4=cfg.vtdm4, p_fac=cfg.p_fac,
+                rf_cutoff=cfg.rf_cutoff, tau=cfg.tau, delt_max=cfg.delt_max)
+    pf = init_pfull(ak, bk)
+    times = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end or not times:
+        S = acoustic_state(st0, init)
+        halo = HostHalo(S, n)
+        t0 = time.perf_counter()
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            odc.acoustic_dynamics(S, ocfg, metrics, dict(da_min=m["da_min"],
+                                                         da_min_c=m["da_min_c"]),
+                                  ak, bk, pf, halo, halo.sync, col, nz)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    cells = 6 * n * n * nz * n_split
+    return {
+        "value": cells / med,
+        "unit": "cell-level-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"numpy oracle AcousticDynamics on 6 x C{n} x {nz} levels, n_split={n_split}, "
+        f"same initial state and namelist, median of {len(times)} calls "
+        f"({med:.2f} s/call), host {os.cpu_count()} cpus visible",
+    }
+
+
+def _timing(lib, name):
+    import ctypes
+
+    cnt, ms, byt = ctypes.c_int64(0), ctypes.c_double(0.0), ctypes.c_double(0.0)
+    rc = lib.fv3_timing_read(ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(byt))
+    if rc != 0:
+        raise RuntimeError(lib.fv3_last_error().decode())
+    return cnt.value, ms.value, byt.value
+
+
+def bench_dyncore(steps: int, warmup: int, roofline_kernel: str):
+    import ctypes
+
+    import torch
+
+    from fv3core_amd import _lib
+    from fv3core_amd._config import DynamicalCoreConfig
+    from fv3core_amd.halo import CubedSphereCommunicator
+    from fv3core_amd.stencils.dyn_core import AcousticDynamics
+    from fv3core_amd.stencils.fv_dynamics import init_pfull
+    from fv3core_amd.utils import gt4py_utils as utils
+    from fv3core_amd.utils.grid import DampingCoefficients, GridData
+    from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
+    from fv3core_amd.workloads import cube_state
+
+    n, nz, n_split = 96, 79, 6
+    init = cube_state(n, nz, seed=0)
+    st0, m = init["state"], init["metric"]
+    ak, bk, ptop = init["ak"], init["bk"], init["ptop"]
+    cfg = DynamicalCoreConfig(npz=nz, n_split=n_split, k_split=1, nord=3, dt_atmos=225)
+    gi = GridIndexing((n, n, nz), 3, True, True, True, True)
+    factory = StencilFactory(StencilConfig(), gi)
+
+    def dev(a, bd):
+        t = utils.empty(a.shape, batch_dims=bd)
+        t.copy_(torch.as_tensor(a))
+        return t
+
+    grid = GridData({k: dev(v, 1) for k, v in m.items()
+                     if isinstance(v, np.ndarray) and v.ndim == 3})
+    damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
+    comm = CubedSphereCommunicator(gi)
+    phis = dev(init["phis"], 1)
+    acoustic = AcousticDynamics(comm, factory, grid, damp, 0, False, False,
+                                cfg.acoustic_dynamics, ak, bk, init_pfull(ak, bk, 1.0e5), phis)
+    state = types.SimpleNamespace(**{k: dev(v, 0) for k, v in st0.items()})
+    state.phis = phis
+    state.wsd = dev(init["wsd"], 1)
+    state.ptop, state.ks, state.mdt, state.n_map = ptop, 0, float(cfg.dt_atmos), 1
+    lib = _lib.lib()
+    for _ in range(warmup):
+        acoustic(state)
+    torch.cuda.synchronize()
+    lib.fv3_timing_arm(roofline_kernel.encode())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        acoustic(state)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cnt, kern_ms, kern_bytes = _timing(lib, roofline_kernel)
+    lib.fv3_timing_arm(None)
+    finite = bool(torch.isfinite(state.pt[:, 3:n + 3, 3:n + 3, :nz]).all().item())
+    cells = 6 * n * n * nz * n_split
+    avg_s = kern_ms / max(cnt, 1) * 1e-3
+    achieved = kern_bytes / max(kern_ms * 1e-3, 1e-30) / 1e9
+    return {
+        "value": steps * cells / wall,
+        "ms_per_step": wall / steps * 1e3,
+        "config": {
+            "workload": f"AcousticDynamics (dyn_core) on 6 cube tiles x {n}x{n}x{nz}, "
+            f"n_split={n_split}, dt=225 s, nonhydrostatic, jet + 1 K pt bump initial state",
+            "tiles": 6, "n": n, "nz": nz, "n_split": n_split,
+            "cells_per_step": cells, "state_finite_after_timed_steps": finite,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": roofline_kernel,
+            "kernel_launches_timed": cnt,
+            "kernel_avg_us": avg_s * 1e6,
+            "algorithmic_bytes_per_launch": kern_bytes / max(cnt, 1),
+        },
+        "dtype": "f64",
     }
 
 
@@ -168,8 +322,17 @@ def main():
                                    "members share one MI355X); --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    res = bench_fvtp2d(args.steps, args.warmup, args.dtype, args.copy_calibration)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
+    if args.workload == "dyncore":
+        steps = 10 if args.steps is None else args.steps
+        warmup = 2 if args.warmup is None else args.warmup
+        res = bench_dyncore(steps, warmup, args.roofline_kernel)
+        cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
+    else:
+        steps = 200 if args.steps is None else args.steps
+        warmup = 20 if args.warmup is None else args.warmup
+        res = bench_fvtp2d(steps, warmup, args.dtype, args.copy_calibration)
+        cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_seconds)
+    args.steps, args.warmup = steps, warmup
     line = {
         "metric": "cell-level-updates/s",
         "value": res["value"],

@@ -9,6 +9,7 @@
 // the two flux fields.  HBM traffic is the compulsory reads of q, the Courant
 // numbers and the area/mass fluxes plus the two flux writes.
 #include "common.hpp"
+#include "timing.hpp"
 #include "ppm.hpp"
 
 namespace fv3 {
@@ -218,6 +219,13 @@ static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T*
   const int64_t launched = (total + kNumXcd - 1) / kNumXcd * kNumXcd;
   dim3 grid((unsigned)launched), block(TR_NT);
   hipStream_t s = as_stream(stream);
+  // algorithmic bytes: q (and its y-corner copy when it is a separate
+  // array), crx, cry, both area fluxes and the optional mass fluxes read
+  // once, both flux fields written once, over the compute cells
+  const double nfld = 7.0 + (q_y != q_x ? 1.0 : 0.0) + (xmf ? 1.0 : 0.0) + (ymf ? 1.0 : 0.0);
+  TimedLaunch timed("fvtp2d_kernel", s,
+                    nfld * sizeof(T) * (double)(d.iec - d.isc + 1) * (d.jec - d.jsc + 1) *
+                        d.nk * d.ntiles);
   switch (hord) {
     case 5:
       hipLaunchKernelGGL((fvtp2d_kernel<T, 5>), grid, block, 0, s, d, q_x, q_y, crx, cry,

@@ -9,6 +9,7 @@
 // Column intermediates live in a caller-provided workspace of field-shaped
 // slabs (the reference's component-owned temporaries).
 #include "common.hpp"
+#include "timing.hpp"
 
 namespace fv3 {
 
@@ -411,6 +412,9 @@ static int map_single_launch(const fv3_domain* dom, int kord, int iv, int i1, in
   RemapArgs a{akord, iv, km, i1, j1, nib, njb, qmin, work_slab};
   Dom d = to_dom(dom);
   dim3 grid((unsigned)(((int64_t)nib * njb + 255) / 256), d.ntiles);
+  // q1, pe1, pe2 read and q1 written once per cell-level of the box
+  TimedLaunch timed("map_single_kernel", as_stream(stream),
+                    4.0 * sizeof(T) * (double)nib * njb * km * d.ntiles);
   hipLaunchKernelGGL((map_single_kernel<T>), grid, dim3(256), 0, as_stream(stream), d, a, q1, pe1,
                      pe2, qs, work);
   FV3_HIP_CHECK_LAUNCH(who);

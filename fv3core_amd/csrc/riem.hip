@@ -11,6 +11,7 @@
 // later sweep reads back live in a caller-provided workspace of field-shaped
 // slabs; cheap ones (gm, peg, peln) are recomputed instead of stored.
 #include "common.hpp"
+#include "timing.hpp"
 
 namespace fv3 {
 
@@ -271,6 +272,9 @@ int32_t fv3_riem_solver_c_f64(const fv3_domain* dom, double dt2, double ptop, do
   Box2 bx{i1, j1, i2 - i1 + 1, j2 - j1 + 1};
   Dom d = to_dom(dom);
   dim3 grid((unsigned)(((int64_t)bx.nib * bx.njb + 255) / 256), d.ntiles);
+  // cappa, ptc, q_con, delpc, gz, w3 read; gz, pef written (per cell-level)
+  TimedLaunch timed("riem_c_kernel", as_stream(stream),
+                    8.0 * sizeof(double) * bx.nib * bx.njb * (double)dom->nk * d.ntiles);
   hipLaunchKernelGGL(riem_c_kernel, grid, dim3(256), 0, as_stream(stream), d, bx, dt2, ptop,
                      p_fac, cappa, hs, ws, ptc, q_con, delpc, gz, pef, w3, work, work_slab);
   FV3_HIP_CHECK_LAUNCH(who);
@@ -312,6 +316,11 @@ int32_t fv3_riem_solver3_f64(const fv3_domain* dom, int32_t last_call, double dt
   o.rgrav = 1.0 / kGrav;
   Dom d = to_dom(dom);
   dim3 grid((unsigned)(((int64_t)bx.nib * bx.njb + 255) / 256), d.ntiles);
+  // cappa, delz, q_con, delp, pt, zh, pe, w read; zh, pe, ppe, pk3, delz, w
+  // written, plus pk and peln on the last call (per cell-level)
+  TimedLaunch timed("riem3_kernel", as_stream(stream),
+                    (14.0 + 2.0 * o.last_call) * sizeof(double) * bx.nib * bx.njb *
+                        (double)dom->nk * d.ntiles);
   hipLaunchKernelGGL(riem3_kernel, grid, dim3(256), 0, as_stream(stream), d, bx, o, cappa, zs,
                      wsd, delz, q_con, delp, pt, zh, pe, ppe, pk3, pk, peln, w, work, work_slab);
   FV3_HIP_CHECK_LAUNCH(who);

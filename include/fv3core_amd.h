@@ -57,6 +57,17 @@ const char* fv3_last_error(void);
 /* Number of gfx950 devices visible (0 on a machine without a GPU). */
 int32_t fv3_device_count(void);
 
+/* Launch timing (no counterpart in the reference; for benchmarks):
+ * fv3_timing_arm(name) makes every later launch of the kernel called
+ * `name` record HIP events around itself on its stream (NULL or "" stops
+ * it); fv3_timing_read waits for those events and returns the number of
+ * timed launches, their summed duration and their summed algorithmic HBM
+ * bytes (inputs read once, outputs written once), then starts a new count.
+ * Timed kernels: fvtp2d_kernel, map_single_kernel, riem_c_kernel,
+ * riem3_kernel. */
+int32_t fv3_timing_arm(const char* kernel_name);
+int32_t fv3_timing_read(int64_t* count, double* total_ms, double* total_bytes);
+
 /*
  * CopyCornersXY.__call__ (fv3core/utils/corners.py:64-117, stencil
  * copy_corners_xy_stencil_defn corners.py:425-540):
