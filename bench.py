@@ -50,6 +50,8 @@ def _parse():
                    choices=["fvtp2d_kernel", "riem3_kernel", "riem_c_kernel",
                             "map_single_kernel"])
     p.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    p.add_argument("--graph", action="store_true",
+                   help="dyncore: also time the step replayed from a captured HIP graph")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--copy-calibration", action="store_true",
@@ -149,7 +151,7 @@ def _timing(lib, name):
     return cnt.value, ms.value, byt.value
 
 
-def bench_dyncore(steps: int, warmup: int, roofline_kernel: str):
+def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = False):
     import ctypes
 
     import torch
@@ -184,7 +186,7 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str):
     phis = dev(init["phis"], 1)
     acoustic = AcousticDynamics(comm, factory, grid, damp, 0, False, False,
                                 cfg.acoustic_dynamics, ak, bk, init_pfull(ak, bk, 1.0e5), phis)
-    state = types.SimpleNamespace(**{k: dev(v, 0) for k, v in st0.items()})
+    state = types.SimpleNamespace(**{k: dev(v, None) for k, v in st0.items()})
     state.phis = phis
     state.wsd = dev(init["wsd"], 1)
     state.ptop, state.ks, state.mdt, state.n_map = ptop, 0, float(cfg.dt_atmos), 1
@@ -202,12 +204,25 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str):
     cnt, kern_ms, kern_bytes = _timing(lib, roofline_kernel)
     lib.fv3_timing_arm(None)
     finite = bool(torch.isfinite(state.pt[:, 3:n + 3, 3:n + 3, :nz]).all().item())
+    graph_wall = None
+    if graph:
+        acoustic._use_graph = True
+        for _ in range(max(warmup, 0) + 2):
+            acoustic(state)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            acoustic(state)
+        torch.cuda.synchronize()
+        graph_wall = time.perf_counter() - t0
     cells = 6 * n * n * nz * n_split
     avg_s = kern_ms / max(cnt, 1) * 1e-3
     achieved = kern_bytes / max(kern_ms * 1e-3, 1e-30) / 1e9
     return {
-        "value": steps * cells / wall,
-        "ms_per_step": wall / steps * 1e3,
+        "value": steps * cells / (graph_wall if graph_wall else wall),
+        "ms_per_step": (graph_wall if graph_wall else wall) / steps * 1e3,
+        "eager_ms_per_step": wall / steps * 1e3,
+        "graph_ms_per_step": graph_wall / steps * 1e3 if graph_wall else None,
         "config": {
             "workload": f"AcousticDynamics (dyn_core) on 6 cube tiles x {n}x{n}x{nz}, "
             f"n_split={n_split}, dt=225 s, nonhydrostatic, jet + 1 K pt bump initial state",
@@ -325,7 +340,7 @@ def main():
     if args.workload == "dyncore":
         steps = 10 if args.steps is None else args.steps
         warmup = 2 if args.warmup is None else args.warmup
-        res = bench_dyncore(steps, warmup, args.roofline_kernel)
+        res = bench_dyncore(steps, warmup, args.roofline_kernel, args.graph)
         cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
     else:
         steps = 200 if args.steps is None else args.steps
@@ -350,6 +365,9 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
     }
+    for extra in ("eager_ms_per_step", "graph_ms_per_step"):
+        if res.get(extra) is not None:
+            line[extra] = res[extra]
     if "copy_calibration_bytes" in res:
         line["copy_calibration_bytes"] = res["copy_calibration_bytes"]
     print(json.dumps(line))
