@@ -338,8 +338,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload == "dyncore":
-        steps = 10 if args.steps is None else args.steps
-        warmup = 2 if args.warmup is None else args.warmup
+        steps = 20 if args.steps is None else args.steps
+        warmup = 3 if args.warmup is None else args.warmup
         res = bench_dyncore(steps, warmup, args.roofline_kernel, args.graph)
         cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
     else:
