@@ -12,6 +12,12 @@ T = 288 K lapsing to 216 K, u = 35 sin^2(2 lat), seeded 1 K pt bump, w = 0).
 One step is one call; cell-level updates per step = 6 * 96 * 96 * 79 * 6
 (every cell advanced through every acoustic substep).
 
+Workload "fv_dynamics": DynamicalCore.step_dynamics on six C128 tiles
+128x128x79 (the same initial state at C128 plus 8 smooth tracers in
+[0, 1e-2]): 6 acoustic substeps, tracer advection (hord 8) and the remap
+with kord 10; cell-level updates per step = 6 * 128 * 128 * 79 (the acoustic
+count, 6x that, is given beside it).
+
 Workload "fvtp2d": FiniteVolumeTransport (fv_tp_2d, hord=8, area fluxes) on
 6 tiles of 96x96x79, FP64, one fused kernel per step; cell-level updates per
 step = 6 * 96 * 96 * 79.
@@ -45,7 +51,8 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", default="dyncore", choices=["dyncore", "fvtp2d"])
+    p.add_argument("--workload", default="dyncore",
+                   choices=["dyncore", "fv_dynamics", "fvtp2d"])
     p.add_argument("--roofline-kernel", default="fvtp2d_kernel",
                    choices=["fvtp2d_kernel", "riem3_kernel", "riem_c_kernel",
                             "map_single_kernel"])
@@ -141,6 +148,19 @@ def cpu_baseline_dyncore(seconds: float):
     }
 
 
+def _copy_calibration():
+    """A plain device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of a profiled run (256 MiB read + 256 MiB written)."""
+    import torch
+
+    a = torch.ones(256 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return a.numel() * 8
+
+
 def _timing(lib, name):
     import ctypes
 
@@ -151,7 +171,8 @@ def _timing(lib, name):
     return cnt.value, ms.value, byt.value
 
 
-def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = False):
+def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = False,
+                  copy_calibration: bool = False):
     import ctypes
 
     import torch
@@ -215,6 +236,7 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
             acoustic(state)
         torch.cuda.synchronize()
         graph_wall = time.perf_counter() - t0
+    copy_bytes = _copy_calibration() if copy_calibration else None
     cells = 6 * n * n * nz * n_split
     avg_s = kern_ms / max(cnt, 1) * 1e-3
     achieved = kern_bytes / max(kern_ms * 1e-3, 1e-30) / 1e9
@@ -239,6 +261,89 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
             "kernel": roofline_kernel,
             "kernel_launches_timed": cnt,
             "kernel_avg_us": avg_s * 1e6,
+            "algorithmic_bytes_per_launch": kern_bytes / max(cnt, 1),
+        },
+        "dtype": "f64",
+        "copy_calibration_bytes": copy_bytes,
+    }
+
+
+def bench_fv_dynamics(steps: int, warmup: int, roofline_kernel: str):
+    """DynamicalCore.step_dynamics on six C128 tiles x 79 levels: n_split = 6
+    acoustic substeps, 8 tracers advected (hord_tr = 8), remap with kord 10."""
+    import torch
+
+    from fv3core_amd import _lib
+    from fv3core_amd._config import DynamicalCoreConfig
+    from fv3core_amd.halo import CubedSphereCommunicator
+    from fv3core_amd.stencils.fv_dynamics import DynamicalCore
+    from fv3core_amd.utils import gt4py_utils as utils
+    from fv3core_amd.utils.grid import DampingCoefficients, GridData
+    from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
+    from fv3core_amd.workloads import cube_state, cube_tracers
+
+    n, nz, n_split = 128, 79, 6
+    init = cube_state(n, nz, seed=0)
+    st0, m = init["state"], init["metric"]
+    ak, bk, ptop = init["ak"], init["bk"], init["ptop"]
+    cfg = DynamicalCoreConfig(npz=nz, n_split=n_split, k_split=1, nord=3, dt_atmos=225,
+                              kord_tm=-10, kord_tr=10, kord_wz=10, kord_mt=10, hord_tr=8)
+    gi = GridIndexing((n, n, nz), 3, True, True, True, True)
+    factory = StencilFactory(StencilConfig(), gi)
+
+    def dev(a, bd):
+        t = utils.empty(a.shape, batch_dims=bd)
+        t.copy_(torch.as_tensor(a))
+        return t
+
+    grid = GridData({k: dev(v, 1) for k, v in m.items()
+                     if isinstance(v, np.ndarray) and v.ndim == 3})
+    damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
+    comm = CubedSphereCommunicator(gi)
+    phis = dev(init["phis"], 1)
+    core = DynamicalCore(comm, grid, factory, damp, cfg, ak, bk, phis)
+    state = {k: dev(v, None) for k, v in st0.items() if k != "cappa"}
+    state.update({k: dev(v, None) for k, v in cube_tracers(st0["delp"].shape).items()})
+    state["phis"] = phis
+    state["ps"] = dev(st0["pe"][..., nz].copy(), 1)
+    bdt = float(cfg.dt_atmos)
+    lib = _lib.lib()
+    for _ in range(warmup):
+        core.step_dynamics(state, 0.0, False, bdt, ptop, n_split, 0)
+    torch.cuda.synchronize()
+    lib.fv3_timing_arm(roofline_kernel.encode())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        core.step_dynamics(state, 0.0, False, bdt, ptop, n_split, 0)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cnt, kern_ms, kern_bytes = _timing(lib, roofline_kernel)
+    lib.fv3_timing_arm(None)
+    finite = bool(torch.isfinite(state["pt"][:, 3:n + 3, 3:n + 3, :nz]).all().item())
+    cells = 6 * n * n * nz
+    achieved = kern_bytes / max(kern_ms * 1e-3, 1e-30) / 1e9
+    return {
+        "value": steps * cells / wall,
+        "ms_per_step": wall / steps * 1e3,
+        "config": {
+            "workload": f"DynamicalCore.step_dynamics on 6 cube tiles x {n}x{n}x{nz}: "
+            f"n_split={n_split} acoustic substeps, 8 tracers (hord_tr=8), remap kord=10 "
+            "with moist_cv, dt=225 s",
+            "tiles": 6, "n": n, "nz": nz, "n_split": n_split, "tracers": 8,
+            "cells_per_step": cells, "acoustic_cells_per_step": cells * n_split,
+            "state_finite_after_timed_steps": finite,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": roofline_kernel,
+            "kernel_launches_timed": cnt,
+            "kernel_avg_us": kern_ms / max(cnt, 1) * 1e3,
             "algorithmic_bytes_per_launch": kern_bytes / max(cnt, 1),
         },
         "dtype": "f64",
@@ -340,8 +445,14 @@ def main():
     if args.workload == "dyncore":
         steps = 20 if args.steps is None else args.steps
         warmup = 3 if args.warmup is None else args.warmup
-        res = bench_dyncore(steps, warmup, args.roofline_kernel, args.graph)
+        res = bench_dyncore(steps, warmup, args.roofline_kernel, args.graph,
+                            args.copy_calibration)
         cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
+    elif args.workload == "fv_dynamics":
+        steps = 5 if args.steps is None else args.steps
+        warmup = 1 if args.warmup is None else args.warmup
+        res = bench_fv_dynamics(steps, warmup, args.roofline_kernel)
+        cpu = None
     else:
         steps = 200 if args.steps is None else args.steps
         warmup = 20 if args.warmup is None else args.warmup
@@ -368,7 +479,7 @@ def main():
     for extra in ("eager_ms_per_step", "graph_ms_per_step"):
         if res.get(extra) is not None:
             line[extra] = res[extra]
-    if "copy_calibration_bytes" in res:
+    if res.get("copy_calibration_bytes"):
         line["copy_calibration_bytes"] = res["copy_calibration_bytes"]
     print(json.dumps(line))
 

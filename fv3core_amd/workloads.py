@@ -348,3 +348,27 @@ def cube_state(n: int = 96, nz: int = 79, seed: int = 0, ps: float = 1.0e5):
                  diss_estd=z.copy(), pe=pe, pk=pk, peln=peln, pkz=pkz, zh=zh)
     return dict(state=state, ak=ak, bk=bk, ptop=ptop, metric=m,
                 phis=np.zeros((6, ni, ni)), wsd=np.zeros((6, ni, ni)))
+
+
+def cube_tracers(shape, seed: int = 5000, names=None):
+    """Nine tracer fields of a batched storage shape (6, I, J, K): smooth seeded
+    fields in [0, 1e-2] kg/kg (tracer t from rng(seed + 1 + t)), the last
+    (cloud fraction) in [0, 1]."""
+    from .constants import TRACER_VARIABLES
+
+    names = list(names or TRACER_VARIABLES)
+    nt, ni, nj, nk = shape
+    ii = np.arange(ni)[None, :, None, None] / ni
+    jj = np.arange(nj)[None, None, :, None] / nj
+    kk = np.arange(nk)[None, None, None, :] / nk
+    tt = np.arange(nt)[:, None, None, None]
+    out = {}
+    for n, name in enumerate(names):
+        rng = np.random.default_rng(seed + 1 + n)
+        p = rng.uniform(0.0, 2.0 * np.pi, size=4)
+        f = (np.sin(2 * np.pi * ii + p[0] + tt) * np.cos(2 * np.pi * jj + p[1])
+             * np.cos(np.pi * kk + p[2]) + 0.5 * np.sin(2 * np.pi * (ii + jj) + p[3]))
+        f = (f - f.min()) / (f.max() - f.min())
+        scale = 1.0 if name == "qcld" else 1.0e-2
+        out[name] = np.ascontiguousarray(scale * f)
+    return out
