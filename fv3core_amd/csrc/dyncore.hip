@@ -44,6 +44,35 @@ __device__ __forceinline__ T fill2_view(const T* q, const Dom& d, int i, int j, 
 }
 
 
+// Levels per chunk of a column sweep: a chunk's loads are issued together
+// before the recurrence runs over them, so a column waits for memory once
+// per chunk rather than once per level (the sweeps are latency-bound: a
+// cube of C96 tiles has under one wave of columns per SIMD).
+constexpr int KC = 8;
+
+// h[k] = max(h[k], h[k+1] + DZ_MIN) for k = nk-2 .. 0 (BACKWARD), given
+// h[nk-1] = bottom.
+template <typename T>
+__device__ __forceinline__ void floor_column_up(T* __restrict__ h, int64_t sk, int nk,
+                                                T bottom) {
+  T above = bottom;
+  for (int k0 = nk - 2; k0 >= 0; k0 -= KC) {
+    T v[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u)
+      if (k0 - u >= 0) v[u] = h[(int64_t)(k0 - u) * sk];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      if (k0 - u >= 0) {
+        const T kp1 = above + T(kDzMin);
+        const T w = v[u] > kp1 ? v[u] : kp1;
+        h[(int64_t)(k0 - u) * sk] = w;
+        above = w;
+      }
+    }
+  }
+}
+
 // ----------------------------------------------------------------- updatedzc
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -92,15 +121,7 @@ __global__ void __launch_bounds__(256)
     last = v;
   }
   ws[o2] = (zs[o2] - last) * rdt;
-  T above = last;
-  for (int k = nk - 2; k >= 0; --k) {
-    const int64_t o = at3(d, i, j, k, t);
-    const T kp1 = above + T(kDzMin);
-    T v = gzn[o];
-    v = v > kp1 ? v : kp1;
-    gzn[o] = v;
-    above = v;
-  }
+  floor_column_up(gzn + at3(d, i, j, 0, t), d.sk, nk, last);
 }
 
 // ----------------------------------------------------------------- updatedzd
@@ -126,9 +147,21 @@ __global__ void __launch_bounds__(256)
     prev = (xt1 * qc[base] + qc[base + sk]) / beta[0];
     qi[base] = prev;
   }
-  for (int k = 1; k < nk - 1; ++k) {
-    prev = (T(3) * (qc[base + (k - 1) * sk] + gk[k] * qc[base + k * sk]) - prev) / beta[k];
-    qi[base + k * sk] = prev;
+  T q_prev = qc[base];  // qc[k - 1]
+  for (int k0 = 1; k0 < nk - 1; k0 += KC) {
+    T qv[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u)
+      if (k0 + u < nk - 1) qv[u] = qc[base + (k0 + u) * sk];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk - 1) {
+        prev = (T(3) * (q_prev + gk[k] * qv[u]) - prev) / beta[k];
+        qi[base + k * sk] = prev;
+        q_prev = qv[u];
+      }
+    }
   }
   {
     const int k = nk - 1;
@@ -139,10 +172,20 @@ __global__ void __launch_bounds__(256)
     prev = (xt1 * qc[base + (k - 1) * sk] + qc[base + (k - 2) * sk] - a_bot * prev) / xt2;
     qi[base + k * sk] = prev;
   }
-  for (int k = nk - 2; k >= 0; --k) {
-    const T v = qi[base + k * sk] - gamma[k] * prev;
-    qi[base + k * sk] = v;
-    prev = v;
+  for (int k0 = nk - 2; k0 >= 0; k0 -= KC) {
+    T qv[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u)
+      if (k0 - u >= 0) qv[u] = qi[base + (k0 - u) * sk];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        const T v = qv[u] - gamma[k] * prev;
+        qi[base + k * sk] = v;
+        prev = v;
+      }
+    }
   }
 }
 
@@ -169,16 +212,9 @@ __global__ void __launch_bounds__(256)
     const T adv = (h[o] * a + fx[o] - fx[o + 1] + fy[o] - fy[o + sj]) / after;
     h[o] = adv + (gxd[o] - gxd[o + 1] + gyd[o] - gyd[o + sj]) / a;
   }
-  T above = h[at3(d, i, j, nk - 1, t)];
-  ws[o2] = (zs[o2] - above) / dt;
-  for (int k = nk - 2; k >= 0; --k) {
-    const int64_t o = at3(d, i, j, k, t);
-    const T other = above + T(kDzMin);
-    T v = h[o];
-    v = v > other ? v : other;
-    h[o] = v;
-    above = v;
-  }
+  const T bottom = h[at3(d, i, j, nk - 1, t)];
+  ws[o2] = (zs[o2] - bottom) / dt;
+  floor_column_up(h + at3(d, i, j, 0, t), d.sk, nk, bottom);
 }
 
 // ------------------------------------------------------------------- rings
@@ -195,9 +231,19 @@ __global__ void __launch_bounds__(256)
   T pe = ptop;
   const int64_t base = at3(d, i, j, 0, t), sk = d.sk;
   if (mode == 1) out[base] = ptop;
-  for (int k = 1; k < d.nk; ++k) {
-    pe = pe + delp[base + (k - 1) * sk];
-    out[base + k * sk] = mode == 0 ? pow(pe, akap) : pe;
+  for (int k0 = 1; k0 < d.nk; k0 += KC) {
+    T dp[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u)
+      if (k0 + u < d.nk) dp[u] = delp[base + (k0 + u - 1) * sk];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < d.nk) {
+        pe = pe + dp[u];
+        out[base + k * sk] = mode == 0 ? pow(pe, akap) : pe;
+      }
+    }
   }
 }
 

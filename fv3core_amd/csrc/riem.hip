@@ -10,6 +10,13 @@
 // i, so each k-indexed access is coalesced.  Column intermediates that a
 // later sweep reads back live in a caller-provided workspace of field-shaped
 // slabs; cheap ones (gm, peg, peln) are recomputed instead of stored.
+//
+// A C96 cube has ~55k columns, under one wave per SIMD, so the sweeps are
+// bound by memory latency rather than bandwidth.  Every sweep therefore
+// walks k in chunks of KC levels: the chunk's loads (which never depend on
+// the values the sweep carries from level to level) are issued together
+// into registers, then the recurrence runs on them and the results are
+// stored.  Each chunk waits for memory once instead of once per level.
 #include "common.hpp"
 #include "timing.hpp"
 
@@ -18,92 +25,274 @@ namespace fv3 {
 constexpr double kRdgas = 287.05;
 constexpr double kGrav = 9.80665;
 constexpr double kCpAir = 1004.6;
-
-template <typename T>
-struct KCol {
-  T* p;
-  int64_t sk;
-  __device__ __forceinline__ T& operator[](int k) const { return p[(int64_t)k * sk]; }
-};
+constexpr int KC = 8;  // levels per chunk of a column sweep
 
 // Arrays sim1 works on (sim1_solver.py:126-139): w and dz are in/out, pe is
 // out; gm is 1 / (1 - cp3) of the same column (recomputed on the fly).
+// All pointers address level 0 of one column; levels are sk apart.
 struct Sim1Cols {
-  KCol<double> w, dm, dz, ptr, pm, pe, pem, cp3;
-  KCol<double> w1, gam, pp, aa;  // workspace
+  double *w, *dm, *dz, *ptr, *pm, *pe, *pem, *cp3;
+  double *w1, *gam, *pp, *aa;  // workspace
 };
 
-__device__ void sim1_column(const Sim1Cols& c, int nk, double wsr, double dt, double p_fac) {
+__device__ __forceinline__ double gm_of(double cp3) { return 1.0 / (1.0 - cp3); }
+
+__device__ void sim1_column(const Sim1Cols& c, int nk, int64_t sk, double wsr, double dt,
+                            double p_fac) {
   const int nz = nk - 1;
   const double t1g = 2.0 * dt * dt;
   const double rdt = 1.0 / dt;
-  auto gm = [&](int k) -> double { return 1.0 / (1.0 - c.cp3[k]); };
-  // pe = exp(gm*log(-dm/dz*Rd*ptr)) - pm; w1 = w
-  for (int k = 0; k < nz; ++k) {
-    c.pe[k] = exp(gm(k) * log(-c.dm[k] / c.dz[k] * kRdgas * c.ptr[k])) - c.pm[k];
-    c.w1[k] = c.w[k];
-  }
-  auto g_rat = [&](int k) -> double { return c.dm[k] / c.dm[k + 1]; };
-  auto bb = [&](int k) -> double { return k < nz - 1 ? 2.0 * (1.0 + g_rat(k)) : 2.0; };
-  auto dd = [&](int k) -> double {
-    return k < nz - 1 ? 3.0 * (c.pe[k] + g_rat(k) * c.pe[k + 1]) : 3.0 * c.pe[k];
-  };
-  // forward elimination for pp (pp[0] = 0, pp[1] = dd[0]/bb[0])
-  double bet_prev = bb(0);  // bet[0]
-  c.pp[0] = 0.0;
-  c.pp[1] = dd(0) / bet_prev;
-  for (int k = 1; k < nz; ++k) {
-    const double g = g_rat(k - 1) / bet_prev;
-    c.gam[k] = g;
-    const double bet_k = bb(k) - g;
-    // pp[k+1] = (dd[k] - pp[k]) / bet[k]
-    c.pp[k + 1] = (dd(k) - c.pp[k]) / bet_k;
-    bet_prev = bet_k;
-  }
-  // back substitution and aa
-  for (int k = nz - 1; k >= 1; --k) {
-    const double ppk = c.pp[k] - c.gam[k] * c.pp[k + 1];
-    c.pp[k] = ppk;
-    c.aa[k] = t1g * 0.5 * (gm(k - 1) + gm(k)) / (c.dz[k - 1] + c.dz[k]) * (c.pem[k] + ppk);
-  }
-  // w tridiagonal (gam reused)
-  double bet = c.dm[0] - c.aa[1];
-  c.w[0] = (c.dm[0] * c.w1[0] + dt * c.pp[1]) / bet;
-  for (int k = 1; k < nz - 1; ++k) {
-    const double g = c.aa[k] / bet;
-    c.gam[k] = g;
-    const double b = c.dm[k] - (c.aa[k] + c.aa[k + 1] + c.aa[k] * g);
-    c.w[k] = (c.dm[k] * c.w1[k] + dt * (c.pp[k + 1] - c.pp[k]) - c.aa[k] * c.w[k - 1]) / b;
-    bet = b;
-  }
-  {
-    const int k = nz - 1;
-    const double p1 = t1g * gm(k) / c.dz[k] * (c.pem[k + 1] + c.pp[k + 1]);
-    const double g = c.aa[k] / bet;
-    c.gam[k] = g;
-    const double b = c.dm[k] - (c.aa[k] + p1 + c.aa[k] * g);
-    c.w[k] = (c.dm[k] * c.w1[k] + dt * (c.pp[k + 1] - c.pp[k]) - p1 * wsr - c.aa[k] * c.w[k - 1]) /
-             b;
-  }
-  for (int k = nz - 2; k >= 0; --k) c.w[k] = c.w[k] - c.gam[k + 1] * c.w[k + 1];
-  // pe from the change of w
-  c.pe[0] = 0.0;
-  for (int k = 1; k < nk; ++k)
-    c.pe[k] = c.pe[k - 1] + c.dm[k - 1] * (c.w[k - 1] - c.w1[k - 1]) * rdt;
-  // p1 (BACKWARD) and the dz update
-  double p1_next = 0.0;
-  for (int k = nz - 1; k >= 0; --k) {
-    double p1;
-    if (k == nz - 1) {
-      p1 = (c.pe[k] + 2.0 * c.pe[k + 1]) * 1.0 / 3.0;
-    } else {
-      p1 = (c.pe[k] + bb(k) * c.pe[k + 1] + g_rat(k) * c.pe[k + 2]) * 1.0 / 3.0 -
-           g_rat(k) * p1_next;
+  double* __restrict__ W = c.w;
+  double* __restrict__ DM = c.dm;
+  double* __restrict__ DZ = c.dz;
+  double* __restrict__ PTR = c.ptr;
+  double* __restrict__ PM = c.pm;
+  double* __restrict__ PE = c.pe;
+  double* __restrict__ PEM = c.pem;
+  double* __restrict__ CP3 = c.cp3;
+  double* __restrict__ W1 = c.w1;
+  double* __restrict__ GAM = c.gam;
+  double* __restrict__ PP = c.pp;
+  double* __restrict__ AA = c.aa;
+  auto at = [sk](int k) { return (int64_t)k * sk; };
+
+  // A. pe = exp(gm*log(-dm/dz*Rd*ptr)) - pm; w1 = w
+  for (int k0 = 0; k0 < nz; k0 += KC) {
+    double dm[KC], dz[KC], pt[KC], cp[KC], pm[KC], w[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nz) {
+        dm[u] = DM[at(k)];
+        dz[u] = DZ[at(k)];
+        pt[u] = PTR[at(k)];
+        cp[u] = CP3[at(k)];
+        pm[u] = PM[at(k)];
+        w[u] = W[at(k)];
+      }
     }
-    p1_next = p1;
-    const double pm = c.pm[k];
-    const double maxp = p_fac * c.dm[k] > p1 + pm ? p_fac * pm : p1 + pm;
-    c.dz[k] = -c.dm[k] * kRdgas * c.ptr[k] * exp((c.cp3[k] - 1.0) * log(maxp));
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nz) {
+        PE[at(k)] = exp(gm_of(cp[u]) * log(-dm[u] / dz[u] * kRdgas * pt[u])) - pm[u];
+        W1[at(k)] = w[u];
+      }
+    }
+  }
+
+  // B. forward elimination for pp (pp[0] = 0, pp[1] = dd[0]/bb[0]) with
+  // bb(k) = 2(1 + g_rat(k)) and dd(k) = 3(pe[k] + g_rat(k) pe[k+1]) below
+  // the last layer, 2 and 3 pe[k] on it; g_rat(k) = dm[k] / dm[k+1]
+  double dm_k = DM[at(0)], pe_k = PE[at(0)];
+  double gr_prev, bet_prev, pp_k;
+  if (nz > 1) {
+    const double dm1 = DM[at(1)], pe1 = PE[at(1)];
+    gr_prev = dm_k / dm1;
+    bet_prev = 2.0 * (1.0 + gr_prev);
+    pp_k = 3.0 * (pe_k + gr_prev * pe1) / bet_prev;
+    dm_k = dm1;
+    pe_k = pe1;
+  } else {
+    gr_prev = 0.0;
+    bet_prev = 2.0;
+    pp_k = 3.0 * pe_k / bet_prev;
+  }
+  PP[at(0)] = 0.0;
+  PP[at(1)] = pp_k;
+  for (int k0 = 1; k0 < nz; k0 += KC) {
+    double dmn[KC], pen[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k + 1 < nz) {
+        dmn[u] = DM[at(k + 1)];
+        pen[u] = PE[at(k + 1)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nz) {
+        const double g = gr_prev / bet_prev;
+        GAM[at(k)] = g;
+        double bb, dd, gr = 0.0;
+        if (k < nz - 1) {
+          gr = dm_k / dmn[u];
+          bb = 2.0 * (1.0 + gr);
+          dd = 3.0 * (pe_k + gr * pen[u]);
+        } else {
+          bb = 2.0;
+          dd = 3.0 * pe_k;
+        }
+        const double bet_k = bb - g;
+        pp_k = (dd - pp_k) / bet_k;
+        PP[at(k + 1)] = pp_k;
+        bet_prev = bet_k;
+        gr_prev = gr;
+        if (k < nz - 1) {
+          dm_k = dmn[u];
+          pe_k = pen[u];
+        }
+      }
+    }
+  }
+
+  // C. back substitution and aa (k = nz-1 .. 1)
+  double pp_next = PP[at(nz)];
+  for (int k0 = nz - 1; k0 >= 1; k0 -= KC) {
+    double pp[KC], gam[KC], cpk[KC], cpm[KC], dzk[KC], dzm[KC], pem[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 1) {
+        pp[u] = PP[at(k)];
+        gam[u] = GAM[at(k)];
+        cpk[u] = CP3[at(k)];
+        cpm[u] = CP3[at(k - 1)];
+        dzk[u] = DZ[at(k)];
+        dzm[u] = DZ[at(k - 1)];
+        pem[u] = PEM[at(k)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 1) {
+        const double ppk = pp[u] - gam[u] * pp_next;
+        PP[at(k)] = ppk;
+        AA[at(k)] = t1g * 0.5 * (gm_of(cpm[u]) + gm_of(cpk[u])) / (dzm[u] + dzk[u]) *
+                    (pem[u] + ppk);
+        pp_next = ppk;
+      }
+    }
+  }
+
+  // D. w tridiagonal, forward (gam reused)
+  double aa_k = AA[at(1)];
+  double pp_c = PP[at(1)];  // pp[k+1] of the level before becomes pp[k]
+  double bet = DM[at(0)] - aa_k;
+  double w_prev = (DM[at(0)] * W1[at(0)] + dt * pp_c) / bet;
+  W[at(0)] = w_prev;
+  for (int k0 = 1; k0 < nz; k0 += KC) {
+    double aan[KC], dm[KC], w1[KC], ppn[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nz) {
+        if (k + 1 < nz) aan[u] = AA[at(k + 1)];
+        dm[u] = DM[at(k)];
+        w1[u] = W1[at(k)];
+        ppn[u] = PP[at(k + 1)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nz) {
+        const double g = aa_k / bet;
+        GAM[at(k)] = g;
+        double wk, b;
+        if (k < nz - 1) {
+          b = dm[u] - (aa_k + aan[u] + aa_k * g);
+          wk = (dm[u] * w1[u] + dt * (ppn[u] - pp_c) - aa_k * w_prev) / b;
+        } else {
+          const double p1 = t1g * gm_of(CP3[at(k)]) / DZ[at(k)] * (PEM[at(k + 1)] + ppn[u]);
+          b = dm[u] - (aa_k + p1 + aa_k * g);
+          wk = (dm[u] * w1[u] + dt * (ppn[u] - pp_c) - p1 * wsr - aa_k * w_prev) / b;
+        }
+        W[at(k)] = wk;
+        w_prev = wk;
+        bet = b;
+        pp_c = ppn[u];
+        if (k + 1 < nz) aa_k = aan[u];
+      }
+    }
+  }
+
+  // E. w back substitution (k = nz-2 .. 0)
+  double w_next = w_prev;  // w[nz-1]
+  for (int k0 = nz - 2; k0 >= 0; k0 -= KC) {
+    double w[KC], gam[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        w[u] = W[at(k)];
+        gam[u] = GAM[at(k + 1)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        w_next = w[u] - gam[u] * w_next;
+        W[at(k)] = w_next;
+      }
+    }
+  }
+
+  // F. pe from the change of w (pe[0] = 0)
+  double pe_run = 0.0;
+  PE[at(0)] = 0.0;
+  for (int k0 = 1; k0 < nk; k0 += KC) {
+    double dm[KC], w[KC], w1[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        dm[u] = DM[at(k - 1)];
+        w[u] = W[at(k - 1)];
+        w1[u] = W1[at(k - 1)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        pe_run = pe_run + dm[u] * (w[u] - w1[u]) * rdt;
+        PE[at(k)] = pe_run;
+      }
+    }
+  }
+
+  // G. p1 (BACKWARD) and the dz update
+  double p1_next = 0.0;
+  for (int k0 = nz - 1; k0 >= 0; k0 -= KC) {
+    double pe0[KC], pe1[KC], pe2[KC], dm0[KC], dm1[KC], pm[KC], pt[KC], cp[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        pe0[u] = PE[at(k)];
+        pe1[u] = PE[at(k + 1)];
+        if (k < nz - 1) {
+          pe2[u] = PE[at(k + 2)];
+          dm1[u] = DM[at(k + 1)];
+        }
+        dm0[u] = DM[at(k)];
+        pm[u] = PM[at(k)];
+        pt[u] = PTR[at(k)];
+        cp[u] = CP3[at(k)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        double p1;
+        if (k == nz - 1) {
+          p1 = (pe0[u] + 2.0 * pe1[u]) * 1.0 / 3.0;
+        } else {
+          const double gr = dm0[u] / dm1[u];
+          const double bb = k < nz - 1 ? 2.0 * (1.0 + gr) : 2.0;
+          p1 = (pe0[u] + bb * pe1[u] + gr * pe2[u]) * 1.0 / 3.0 - gr * p1_next;
+        }
+        p1_next = p1;
+        const double maxp = p_fac * dm0[u] > p1 + pm[u] ? p_fac * pm[u] : p1 + pm[u];
+        DZ[at(k)] = -dm0[u] * kRdgas * pt[u] * exp((cp[u] - 1.0) * log(maxp));
+      }
+    }
   }
 }
 
@@ -126,37 +315,94 @@ __global__ void __launch_bounds__(256)
   const int64_t c2 = (int64_t)t * d.st2 + (int64_t)j * d.sj2 + i;
   const int64_t sk = d.sk;
   const int nk = d.nk, nz = nk - 1;
-  auto S = [&](int s) { return KCol<double>{work + s * slab + c, sk}; };
-  KCol<double> dm = S(0), w = S(1), dz = S(2), pm = S(3), pem = S(4), pe = S(5);
-  KCol<const double> Cap{cappa + c, sk}, Qc{q_con + c, sk}, Dp{delpc + c, sk}, W3{w3 + c, sk};
-  KCol<double> Gz{gz + c, sk}, Pef{pef + c, sk};
+  auto at = [sk](int k) { return (int64_t)k * sk; };
+  double* S[10];
+#pragma unroll
+  for (int s = 0; s < 10; ++s) S[s] = work + s * slab + c;
+  double* __restrict__ DM = S[0];
+  double* __restrict__ W = S[1];
+  double* __restrict__ DZ = S[2];
+  double* __restrict__ PM = S[3];
+  double* __restrict__ PEM = S[4];
+  double* __restrict__ PE = S[5];
+  const double* __restrict__ QC = q_con + c;
+  const double* __restrict__ DP = delpc + c;
+  const double* __restrict__ W3 = w3 + c;
+  double* __restrict__ GZ = gz + c;
+  double* __restrict__ PEF = pef + c;
   // precompute (riem_solver_c.py:13-43)
   double pem_k = ptop, peg_k = ptop;
-  for (int k = 0; k < nk; ++k) {
-    const double dmk = Dp[k];
-    w[k] = W3[k];
-    pem[k] = pem_k;
-    dm[k] = dmk / kGrav;
-    if (k < nz) {
-      dz[k] = Gz[k + 1] - Gz[k];
-      const double peg_n = peg_k + dmk * (1.0 - Qc[k]);
-      pm[k] = (peg_n - peg_k) / log(peg_n / peg_k);
-      peg_k = peg_n;
-      pem_k = pem_k + dmk;
+  for (int k0 = 0; k0 < nk; k0 += KC) {
+    double dp[KC], w[KC], qc[KC], g0[KC], g1[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        dp[u] = DP[at(k)];
+        w[u] = W3[at(k)];
+        if (k < nz) {
+          qc[u] = QC[at(k)];
+          g0[u] = GZ[at(k)];
+          g1[u] = GZ[at(k + 1)];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        const double dmk = dp[u];
+        W[at(k)] = w[u];
+        PEM[at(k)] = pem_k;
+        DM[at(k)] = dmk / kGrav;
+        if (k < nz) {
+          DZ[at(k)] = g1[u] - g0[u];
+          const double peg_n = peg_k + dmk * (1.0 - qc[u]);
+          PM[at(k)] = (peg_n - peg_k) / log(peg_n / peg_k);
+          peg_k = peg_n;
+          pem_k = pem_k + dmk;
+        }
+      }
     }
   }
-  Sim1Cols sc{w, dm, dz, KCol<double>{const_cast<double*>(ptc) + c, sk}, pm, pe, pem,
-              KCol<double>{const_cast<double*>(cappa) + c, sk}, S(6), S(7), S(8), S(9)};
-  (void)Cap;
-  sim1_column(sc, nk, ws[c2], dt2, p_fac);
+  Sim1Cols sc{W,  DM, DZ, const_cast<double*>(ptc) + c, PM, PE, PEM,
+              const_cast<double*>(cappa) + c, S[6], S[7], S[8], S[9]};
+  sim1_column(sc, nk, sk, ws[c2], dt2, p_fac);
   // finalize (riem_solver_c.py:46-64)
-  Pef[0] = ptop;
-  for (int k = 1; k < nk; ++k) Pef[k] = pe[k] + pem[k];
+  PEF[0] = ptop;
+  for (int k0 = 1; k0 < nk; k0 += KC) {
+    double pe[KC], pem[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        pe[u] = PE[at(k)];
+        pem[u] = PEM[at(k)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) PEF[at(k)] = pe[u] + pem[u];
+    }
+  }
   double g = hs[c2];
-  Gz[nz] = g;
-  for (int k = nz - 1; k >= 0; --k) {
-    g = g - dz[k] * kGrav;
-    Gz[k] = g;
+  GZ[at(nz)] = g;
+  for (int k0 = nz - 1; k0 >= 0; k0 -= KC) {
+    double dz[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) dz[u] = DZ[at(k)];
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        g = g - dz[u] * kGrav;
+        GZ[at(k)] = g;
+      }
+    }
   }
 }
 
@@ -181,57 +427,119 @@ __global__ void __launch_bounds__(256)
   const int64_t c2 = (int64_t)t * d.st2 + (int64_t)j * d.sj2 + i;
   const int64_t sk = d.sk;
   const int nk = d.nk, nz = nk - 1;
-  auto S = [&](int s) { return KCol<double>{work + s * slab + c, sk}; };
-  KCol<double> dm = S(0), pm = S(1), pem = S(2), pe_init = S(3);
-  KCol<const double> Qc{q_con + c, sk}, Dp{delp + c, sk};
-  KCol<double> Dz{delz + c, sk}, Zh{zh + c, sk}, Pe{pe + c, sk}, Ppe{ppe + c, sk},
-      Pk3{pk3 + c, sk}, Pk{pk + c, sk}, Peln{peln + c, sk}, W{w + c, sk};
+  auto at = [sk](int k) { return (int64_t)k * sk; };
+  double* S[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) S[s] = work + s * slab + c;
+  double* __restrict__ DM = S[0];
+  double* __restrict__ PM = S[1];
+  double* __restrict__ PEM = S[2];
+  double* __restrict__ PEI = S[3];  // pe on entry
+  const double* __restrict__ QC = q_con + c;
+  const double* __restrict__ DP = delp + c;
+  double* __restrict__ DZ = delz + c;
+  double* __restrict__ ZH = zh + c;
+  double* __restrict__ PE = pe + c;
+  double* __restrict__ PPE = ppe + c;
+  double* __restrict__ PK3 = pk3 + c;
+  double* __restrict__ PK = pk + c;
+  double* __restrict__ PELN = peln + c;
   // precompute (riem_solver3.py:24-68)
   double pem_k = o.ptop, peg_k = o.ptop, pelng_k = o.peln1;
-  for (int k = 0; k < nk; ++k) {
-    const double dmk = Dp[k];
-    pe_init[k] = Pe[k];
-    dm[k] = dmk * o.rgrav;
-    if (k == 0) {
-      pem[0] = o.ptop;
-      Pk3[0] = o.ptk;
+  PEM[0] = o.ptop;
+  PK3[0] = o.ptk;
+  for (int k0 = 0; k0 < nk; k0 += KC) {
+    double dp[KC], pe0[KC], qc[KC], z0[KC], z1[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        dp[u] = DP[at(k)];
+        pe0[u] = PE[at(k)];
+        if (k < nz) {
+          qc[u] = QC[at(k)];
+          z0[u] = ZH[at(k)];
+          z1[u] = ZH[at(k + 1)];
+        }
+      }
     }
-    if (k < nz) {
-      const double pem_n = pem_k + dmk;
-      const double peln_n = log(pem_n);
-      const double peg_n = peg_k + dmk * (1.0 - Qc[k]);
-      const double pelng_n = log(peg_n);
-      pem[k + 1] = pem_n;
-      Pk3[k + 1] = exp(o.kappa * peln_n);
-      pm[k] = (peg_n - peg_k) / (pelng_n - pelng_k);
-      Dz[k] = Zh[k + 1] - Zh[k];
-      pem_k = pem_n;
-      peg_k = peg_n;
-      pelng_k = pelng_n;
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        const double dmk = dp[u];
+        PEI[at(k)] = pe0[u];
+        DM[at(k)] = dmk * o.rgrav;
+        if (k < nz) {
+          const double pem_n = pem_k + dmk;
+          const double peln_n = log(pem_n);
+          const double peg_n = peg_k + dmk * (1.0 - qc[u]);
+          const double pelng_n = log(peg_n);
+          PEM[at(k + 1)] = pem_n;
+          PK3[at(k + 1)] = exp(o.kappa * peln_n);
+          PM[at(k)] = (peg_n - peg_k) / (pelng_n - pelng_k);
+          DZ[at(k)] = z1[u] - z0[u];
+          pem_k = pem_n;
+          peg_k = peg_n;
+          pelng_k = pelng_n;
+        }
+      }
     }
   }
-  Sim1Cols sc{W, dm, Dz, KCol<double>{const_cast<double*>(pt) + c, sk}, pm, Pe, pem,
-              KCol<double>{const_cast<double*>(cappa) + c, sk}, S(4), S(5), S(6), S(7)};
-  sim1_column(sc, nk, wsd[c2], o.dt, o.p_fac);
+  Sim1Cols sc{w + c, DM, DZ, const_cast<double*>(pt) + c, PM, PE, PEM,
+              const_cast<double*>(cappa) + c, S[4], S[5], S[6], S[7]};
+  sim1_column(sc, nk, sk, wsd[c2], o.dt, o.p_fac);
   // finalize (riem_solver3.py:70-103)
-  for (int k = 0; k < nk; ++k) {
-    const double peln_run = k == 0 ? o.peln1 : log(pem[k]);
-    if (o.use_logp) Pk3[k] = peln_run;
-    const double pek = Pe[k];
-    Ppe[k] = o.beta < -0.1 ? pek + pem[k] : pek;
-    if (o.last_call) {
-      Peln[k] = peln_run;
-      Pk[k] = Pk3[k];
-      Pe[k] = pem[k];
-    } else {
-      Pe[k] = pe_init[k];
+  for (int k0 = 0; k0 < nk; k0 += KC) {
+    double pem[KC], pek[KC], pk3[KC], pei[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        pem[u] = PEM[at(k)];
+        pek[u] = PE[at(k)];
+        pk3[u] = PK3[at(k)];
+        pei[u] = PEI[at(k)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        const double peln_run = k == 0 ? o.peln1 : log(pem[u]);
+        double pk3k = pk3[u];
+        if (o.use_logp) {
+          pk3k = peln_run;
+          PK3[at(k)] = pk3k;
+        }
+        PPE[at(k)] = o.beta < -0.1 ? pek[u] + pem[u] : pek[u];
+        if (o.last_call) {
+          PELN[at(k)] = peln_run;
+          PK[at(k)] = pk3k;
+          PE[at(k)] = pem[u];
+        } else {
+          PE[at(k)] = pei[u];
+        }
+      }
     }
   }
   double z = zs[c2];
-  Zh[nz] = z;
-  for (int k = nz - 1; k >= 0; --k) {
-    z = z - Dz[k];
-    Zh[k] = z;
+  ZH[at(nz)] = z;
+  for (int k0 = nz - 1; k0 >= 0; k0 -= KC) {
+    double dz[KC];
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) dz[u] = DZ[at(k)];
+    }
+#pragma unroll
+    for (int u = 0; u < KC; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        z = z - dz[u];
+        ZH[at(k)] = z;
+      }
+    }
   }
 }
 

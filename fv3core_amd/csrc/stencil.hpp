@@ -94,17 +94,49 @@ struct FillFields {
   T* f[kMaxFillFields];
 };
 
-template <typename T>
-__global__ void __launch_bounds__(64)
-    fill_table_kernel(Dom d, FillTable tab, FillFields<T> ff) {
-  const int k = (int)(blockIdx.x * 64 + threadIdx.x);
-  const int t = (int)blockIdx.y;
-  if (k >= d.nk) return;
-  const int64_t base = (int64_t)t * d.st + (int64_t)k * d.sk;
+// Entries grouped into waves: an entry reads or writes a cell that an
+// earlier entry writes or reads only in a later wave, so the entries of one
+// wave are independent and the statement order of the table is kept.
+struct FillWaves {
+  int nlev;
+  signed char level[kMaxFill];
+};
+
+inline FillWaves fill_waves(const FillTable& tab) {
+  FillWaves w;
+  w.nlev = 0;
   for (int e = 0; e < tab.n; ++e) {
-    const T v = ff.f[tab.src_f[e]][base + (int64_t)tab.src_j[e] * d.sj + tab.src_i[e]];
-    ff.f[tab.dst_f[e]][base + (int64_t)tab.dst_j[e] * d.sj + tab.dst_i[e]] =
-        (T)tab.mult[e] * v;
+    int lev = 0;
+    for (int p = 0; p < e; ++p) {
+      const bool raw = tab.src_f[e] == tab.dst_f[p] && tab.src_i[e] == tab.dst_i[p] &&
+                       tab.src_j[e] == tab.dst_j[p];
+      const bool war = tab.dst_f[e] == tab.src_f[p] && tab.dst_i[e] == tab.src_i[p] &&
+                       tab.dst_j[e] == tab.src_j[p];
+      const bool waw = tab.dst_f[e] == tab.dst_f[p] && tab.dst_i[e] == tab.dst_i[p] &&
+                       tab.dst_j[e] == tab.dst_j[p];
+      if ((raw || war || waw) && w.level[p] + 1 > lev) lev = w.level[p] + 1;
+    }
+    w.level[e] = (signed char)lev;
+    if (lev + 1 > w.nlev) w.nlev = lev + 1;
+  }
+  return w;
+}
+
+// One workgroup per (level, tile); thread e applies entry e in its wave.
+template <typename T>
+__global__ void __launch_bounds__(128)
+    fill_table_kernel(Dom d, FillTable tab, FillWaves waves, FillFields<T> ff) {
+  const int k = (int)blockIdx.x;
+  const int t = (int)blockIdx.y;
+  const int e = (int)threadIdx.x;
+  const int64_t base = (int64_t)t * d.st + (int64_t)k * d.sk;
+  for (int w = 0; w < waves.nlev; ++w) {
+    if (e < tab.n && waves.level[e] == w) {
+      const T v = ff.f[tab.src_f[e]][base + (int64_t)tab.src_j[e] * d.sj + tab.src_i[e]];
+      ff.f[tab.dst_f[e]][base + (int64_t)tab.dst_j[e] * d.sj + tab.dst_i[e]] =
+          (T)tab.mult[e] * v;
+    }
+    __syncthreads();
   }
 }
 
@@ -112,8 +144,10 @@ template <typename T>
 inline void launch_fill(hipStream_t s, const Dom& d, const FillTable& tab,
                         const FillFields<T>& ff) {
   if (tab.n == 0 || d.nk <= 0 || d.ntiles <= 0) return;
-  dim3 grid((unsigned)((d.nk + 63) / 64), (unsigned)d.ntiles);
-  hipLaunchKernelGGL((fill_table_kernel<T>), grid, dim3(64), 0, s, d, tab, ff);
+  static_assert(kMaxFill <= 128, "one thread per fill entry");
+  const FillWaves waves = fill_waves(tab);
+  dim3 grid((unsigned)d.nk, (unsigned)d.ntiles);
+  hipLaunchKernelGGL((fill_table_kernel<T>), grid, dim3(128), 0, s, d, tab, waves, ff);
 }
 
 }  // namespace fv3
