@@ -161,6 +161,23 @@ def _copy_calibration():
     return a.numel() * 8
 
 
+def _profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` in `workload` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summaries (copy-calibrated), or None."""
+    import glob
+    import re
+
+    pat = re.compile(rf"^traffic_bytes_per_launch {re.escape(kernel)} {re.escape(workload)}:"
+                     r"\s*([0-9.eE+]+)\s*$")
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = pat.match(line.strip())
+                if m:
+                    return float(m.group(1))
+    return None
+
+
 def _timing(lib, name):
     import ctypes
 
@@ -257,7 +274,7 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": _profiled_traffic(roofline_kernel, "dyncore"),
             "kernel": roofline_kernel,
             "kernel_launches_timed": cnt,
             "kernel_avg_us": avg_s * 1e6,

@@ -13,9 +13,15 @@
 namespace fv3 {
 
 
-constexpr int DN_TI = 32;
-constexpr int DN_TJ = 16;
-constexpr int DN_NT = 256;
+// A workgroup is 4 waves; lane x of every wave owns LDS column x (the block
+// plus its 3-cell halo is exactly one wave wide), and wave y owns rows y,
+// y+4, ... so no index is ever divided.
+constexpr int DN_W = 64;            // LDS columns = TI + 6
+constexpr int DN_TI = DN_W - 6;     // interfaces per block in i
+constexpr int DN_TJ = 16;           // interfaces per block in j
+constexpr int DN_H = DN_TJ + 6;     // LDS rows
+constexpr int DN_NY = 4;            // waves per block
+constexpr int DN_NT = DN_W * DN_NY;
 
 struct Box {
   int i0, i1, j0, j1;  // inclusive absolute bounds
@@ -31,17 +37,21 @@ __global__ void __launch_bounds__(DN_NT)
                     const T* __restrict__ damp, const T* __restrict__ del6_u,
                     const T* __restrict__ del6_v, const T* __restrict__ rarea,
                     T* __restrict__ fx, T* __restrict__ fy, int nbi) {
-  constexpr int TI = DN_TI, TJ = DN_TJ;
-  constexpr int W = TI + 6, H = TJ + 6;
+  constexpr int W = DN_W, H = DN_H, TI = DN_TI, TJ = DN_TJ;
+  constexpr int NR = (H + DN_NY - 1) / DN_NY;  // LDS rows per lane
   __shared__ T sd2[H][W];
   __shared__ T sfx[H][W];
   __shared__ T sfy[H][W];
 
-  const int tid = threadIdx.x;
+  const int li = threadIdx.x;  // LDS column of this lane
+  const int ty = threadIdx.y;  // this wave owns LDS rows ty, ty + 4, ...
+  const int tid = ty * W + li;
   const int bi = blockIdx.x % nbi, bj = blockIdx.x / nbi;
   const int k = blockIdx.y, t = blockIdx.z;
   const int i0 = d.isc + bi * TI, j0 = d.jsc + bj * TJ;
   const int ib = i0 - 3, jb = j0 - 3;  // absolute index of LDS [0][0]
+  const int i = ib + li;               // absolute column of this lane
+  const bool col_ok = i >= 0 && i < d.ni;
   const int64_t off3 = (int64_t)t * d.st + (int64_t)k * d.sk;
   const int64_t off2 = (int64_t)t * d.st2;
   const int band = k < 3 ? k : 3;
@@ -49,108 +59,126 @@ __global__ void __launch_bounds__(DN_NT)
   const bool pos = nord_k > 0;
   const bool use_damp = (mode == FV3_DELN_ADD || mode == FV3_DELN_STORE);
   const T dk = damp[k];
-  const int ni = d.ni, nj = d.nj;
+  const int nj = d.nj;
 
-  // d2 preamble (d2_damp_interval / copy_stencil_interval)
+  // Every global value the passes need at this lane's cells is loaded up
+  // front, so the passes below wait only on LDS and barriers.
+  T qv[NR], dv[NR], du[NR], ra[NR];
   {
     const int g = pos ? nmax : 0;
     const Box b0{d.isc - 1 - g, d.iec + 1 + g, d.jsc - 1 - g, d.jec + 1 + g};
-    for (int n = tid; n < W * H; n += DN_NT) {
-      const int li = n % W, lj = n / W;
-      const int i = ib + li, j = jb + lj;
-      T v = T(0);
-      if (b0.in(i, j) && i >= 0 && i < ni && j >= 0 && j < nj) {
-        const T qv = q[off3 + (int64_t)j * d.sj + i];
-        v = use_damp ? dk * qv : qv;
-      }
-      sd2[lj][li] = v;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int lj = ty + r * DN_NY;
+      const int j = jb + lj;
+      const bool ok = lj < H && col_ok && j >= 0 && j < nj;
+      const int64_t o2 = off2 + (int64_t)j * d.sj2 + i;
+      qv[r] = (ok && b0.in(i, j)) ? q[off3 + (int64_t)j * d.sj + i] : T(0);
+      dv[r] = ok ? del6_v[o2] : T(0);
+      du[r] = ok ? del6_u[o2] : T(0);
+      ra[r] = (ok && pos) ? rarea[o2] : T(0);
+    }
+  }
+  // d2 preamble (d2_damp_interval / copy_stencil_interval)
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int lj = ty + r * DN_NY;
+    if (lj < H) {
+      sd2[lj][li] = use_damp ? dk * qv[r] : qv[r];
       sfx[lj][li] = T(0);
       sfy[lj][li] = T(0);
     }
   }
-  __syncthreads();
+  // does any corner-copy destination fall inside this block's window?
+  bool mine = false;
+  for (int e = tid; e < tab.n; e += DN_NT) {
+    const int di = tab.dst_i[e] - ib, dj = tab.dst_j[e] - jb;
+    mine = mine || (di >= 0 && di < W && dj >= 0 && dj < H);
+  }
+  const bool corners = pos && __syncthreads_or(mine);
+  if (!pos) __syncthreads();
 
   auto corner_copy = [&](int dir) {
     // sources are never corner cells, so the in-place copy has no ordering hazard
     for (int e = tid; e < tab.n; e += DN_NT) {
       if (tab.dir[e] != dir) continue;
-      const int li = tab.dst_i[e] - ib, lj = tab.dst_j[e] - jb;
-      if (li < 0 || li >= W || lj < 0 || lj >= H) continue;
+      const int di = tab.dst_i[e] - ib, dj = tab.dst_j[e] - jb;
+      if (di < 0 || di >= W || dj < 0 || dj >= H) continue;
       const int si = tab.src_i[e] - ib, sj = tab.src_j[e] - jb;
       const T v = (si >= 0 && si < W && sj >= 0 && sj < H) ? sd2[sj][si] : T(0);
-      sd2[lj][li] = v;
+      sd2[dj][di] = v;
     }
   };
   auto flux_x = [&](const Box& bx, T sgnv) {
-    for (int n = tid; n < W * H; n += DN_NT) {
-      const int li = n % W, lj = n / W;
-      const int i = ib + li, j = jb + lj;
-      if (li >= 1 && bx.in(i, j) && i < ni && j < nj) {
-        const T c = del6_v[off2 + (int64_t)j * d.sj2 + i];
-        sfx[lj][li] = sgnv * c * (sd2[lj][li - 1] - sd2[lj][li]);
-      }
+    if (li < 1 || !col_ok) return;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int lj = ty + r * DN_NY;
+      const int j = jb + lj;
+      if (lj < H && bx.in(i, j) && j < nj)
+        sfx[lj][li] = sgnv * dv[r] * (sd2[lj][li - 1] - sd2[lj][li]);
     }
   };
   auto flux_y = [&](const Box& bx, T sgnv) {
-    for (int n = tid; n < W * H; n += DN_NT) {
-      const int li = n % W, lj = n / W;
-      const int i = ib + li, j = jb + lj;
-      if (lj >= 1 && bx.in(i, j) && i < ni && j < nj) {
-        const T c = del6_u[off2 + (int64_t)j * d.sj2 + i];
-        sfy[lj][li] = sgnv * c * (sd2[lj - 1][li] - sd2[lj][li]);
-      }
+    if (!col_ok) return;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const int lj = ty + r * DN_NY;
+      const int j = jb + lj;
+      if (lj >= 1 && lj < H && bx.in(i, j) && j < nj)
+        sfy[lj][li] = sgnv * du[r] * (sd2[lj - 1][li] - sd2[lj][li]);
     }
   };
-
-  if (pos) corner_copy(0);
-  __syncthreads();
-  {
-    const Box bx = pos ? Box{d.isc - nmax, d.iec + 1 + nmax, d.jsc - nmax, d.jec + nmax}
-                       : Box{d.isc, d.iec + 1, d.jsc, d.jec};
-    flux_x(bx, T(1));
-  }
-  __syncthreads();
-  if (pos) corner_copy(1);
-  __syncthreads();
-  {
-    const Box by = pos ? Box{d.isc - nmax, d.iec + nmax, d.jsc - nmax, d.jec + 1 + nmax}
-                       : Box{d.isc, d.iec, d.jsc, d.jec + 1};
-    flux_y(by, T(1));
-  }
-  __syncthreads();
+  // one del-2 flux pass: with corner cells in the window the x and y fluxes
+  // see the x- and y-corner-copied d2 (two more barriers); otherwise both
+  // read the same d2 in one phase
+  auto fluxes = [&](const Box& bx, const Box& by, T sgnv) {
+    if (corners) {
+      corner_copy(0);
+      __syncthreads();
+      flux_x(bx, sgnv);
+      __syncthreads();
+      corner_copy(1);
+      __syncthreads();
+      flux_y(by, sgnv);
+    } else {
+      flux_x(bx, sgnv);
+      flux_y(by, sgnv);
+    }
+    __syncthreads();
+  };
 
   if (pos) {
+    fluxes(Box{d.isc - nmax, d.iec + 1 + nmax, d.jsc - nmax, d.jec + nmax},
+           Box{d.isc - nmax, d.iec + nmax, d.jsc - nmax, d.jec + 1 + nmax}, T(1));
     for (int n = 0; n < nmax; ++n) {
       const int nt = nmax - 1 - n;
       // d2_highorder (delnflux.py:178-181)
       const Box bd{d.isc - nt - 1, d.iec + nt + 1, d.jsc - nt - 1, d.jec + nt + 1};
-      for (int m = tid; m < W * H; m += DN_NT) {
-        const int li = m % W, lj = m / W;
-        const int i = ib + li, j = jb + lj;
-        if (li + 1 < W && lj + 1 < H && bd.in(i, j) && i < ni && j < nj) {
-          const T ra = rarea[off2 + (int64_t)j * d.sj2 + i];
-          sd2[lj][li] = (sfx[lj][li] - sfx[lj][li + 1] + sfy[lj][li] - sfy[lj + 1][li]) * ra;
+      if (li + 1 < W && col_ok) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int lj = ty + r * DN_NY;
+          const int j = jb + lj;
+          if (lj + 1 < H && bd.in(i, j) && j < nj)
+            sd2[lj][li] = (sfx[lj][li] - sfx[lj][li + 1] + sfy[lj][li] - sfy[lj + 1][li]) * ra[r];
         }
       }
       __syncthreads();
-      corner_copy(0);
-      __syncthreads();
-      flux_x(Box{d.isc - nt, d.iec + 1 + nt, d.jsc - nt, d.jec + nt}, T(-1));
-      __syncthreads();
-      corner_copy(1);
-      __syncthreads();
-      flux_y(Box{d.isc - nt, d.iec + nt, d.jsc - nt, d.jec + 1 + nt}, T(-1));
-      __syncthreads();
+      fluxes(Box{d.isc - nt, d.iec + 1 + nt, d.jsc - nt, d.jec + nt},
+             Box{d.isc - nt, d.iec + nt, d.jsc - nt, d.jec + 1 + nt}, T(-1));
     }
+  } else {
+    fluxes(Box{d.isc, d.iec + 1, d.jsc, d.jec}, Box{d.isc, d.iec, d.jsc, d.jec + 1}, T(1));
   }
 
   // apply to the caller's fluxes on the block's interfaces
-  for (int n = tid; n < TI * TJ; n += DN_NT) {
-    const int li = n % TI, lj = n / TI;
-    const int i = i0 + li, j = j0 + lj;
+  if (li < 3 || li >= 3 + TI) return;
+  for (int lj = 3 + ty; lj < 3 + TJ; lj += DN_NY) {
+    const int j = jb + lj;
     const int64_t o = off3 + (int64_t)j * d.sj + i;
     if (i <= d.iec + 1 && j <= d.jec) {
-      const T f2 = sfx[lj + 3][li + 3];
+      const T f2 = sfx[lj][li];
       if (mode == FV3_DELN_ADD) {
         fx[o] = fx[o] + f2;
       } else if (mode == FV3_DELN_MASS) {
@@ -160,7 +188,7 @@ __global__ void __launch_bounds__(DN_NT)
       }
     }
     if (i <= d.iec && j <= d.jec + 1) {
-      const T f2 = sfy[lj + 3][li + 3];
+      const T f2 = sfy[lj][li];
       if (mode == FV3_DELN_ADD) {
         fy[o] = fy[o] + f2;
       } else if (mode == FV3_DELN_MASS) {
@@ -203,7 +231,7 @@ static int delnflux_launch(const fv3_domain* dom, int mode, const int32_t* nord4
   const int nbi = (nfi + DN_TI - 1) / DN_TI, nbj = (nfj + DN_TJ - 1) / DN_TJ;
   dim3 grid(nbi * nbj, d.nk, d.ntiles);
   int4 n4 = make_int4(nord4[0], nord4[1], nord4[2], nord4[3]);
-  hipLaunchKernelGGL((delnflux_kernel<T>), grid, dim3(DN_NT), 0, as_stream(stream), d, mode, n4,
+  hipLaunchKernelGGL((delnflux_kernel<T>), grid, dim3(DN_W, DN_NY), 0, as_stream(stream), d, mode, n4,
                      nmax, tab, q, mass, damp, del6_u, del6_v, rarea, fx, fy, nbi);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
