@@ -22,6 +22,38 @@ constexpr double kB2 = -1.0 / 12.0;
 constexpr double kCc1 = 2.0 / 3.0;
 constexpr double kCc2 = -1.0 / 6.0;
 
+// Runs f(i, j, k, t) on the perimeter of box b only (its first and last
+// rows and columns), for levels [k0, k0 + nk) of every tile.
+template <typename F>
+__global__ void __launch_bounds__(256) perimeter_kernel(Box b, int k0, F f) {
+  const int nbi = b.i1 - b.i0 + 1, nbj = b.j1 - b.j0 + 1;
+  const int per = 2 * nbi + 2 * (nbj - 2);
+  const int n = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (n >= per) return;
+  int i, j;
+  if (n < nbi) {
+    i = b.i0 + n;
+    j = b.j0;
+  } else if (n < 2 * nbi) {
+    i = b.i0 + n - nbi;
+    j = b.j1;
+  } else {
+    const int m = n - 2 * nbi;
+    i = m < nbj - 2 ? b.i0 : b.i1;
+    j = b.j0 + 1 + (m < nbj - 2 ? m : m - (nbj - 2));
+  }
+  f(i, j, k0 + (int)blockIdx.y, (int)blockIdx.z);
+}
+
+template <typename F>
+inline void launch_perimeter(hipStream_t s, const Dom& d, Box b, int k0, int nk, F f) {
+  const int nbi = b.i1 - b.i0 + 1, nbj = b.j1 - b.j0 + 1;
+  if (nbi <= 0 || nbj < 3 || nk <= 0 || d.ntiles <= 0) return;
+  const int per = 2 * nbi + 2 * (nbj - 2);
+  dim3 grid((unsigned)((per + 255) / 256), (unsigned)nk, (unsigned)d.ntiles);
+  hipLaunchKernelGGL((perimeter_kernel<F>), grid, dim3(256), 0, s, b, k0, f);
+}
+
 // great_circle_dist (a2b_ord4.py:37-41)
 template <typename T>
 __device__ __forceinline__ T gc_dist(T p1a, T p1b, T p2a, T p2b) {
@@ -109,9 +141,9 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
   const bool so = has_edge(dom->j_start), no = has_edge(dom->j_end);
   const int js2 = so ? d.jsc + 1 : d.jsc, je1 = no ? d.jec : d.jec + 1;
   const int is2 = w ? d.isc + 1 : d.isc, ie1 = e ? d.iec : d.iec + 1;
-  // 1: corners and edge lines, over the box [isc, iec+1]^2 restricted to them
+  // 1: corners and edge lines: only the perimeter of [isc, iec+1]^2 can hold them
   const Box bx{d.isc, d.iec + 1, d.jsc, d.jec + 1};
-  launch_box(s, d, bx, k0, nk, [=] __device__(int i, int j, int k, int t) {
+  launch_perimeter(s, d, bx, k0, nk, [=] __device__(int i, int j, int k, int t) {
     const int64_t o = at3(d, i, j, k, t);
     const int64_t o2 = at2(d, i, j, t);
     T val;
