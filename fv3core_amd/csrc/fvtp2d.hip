@@ -14,8 +14,8 @@
 
 namespace fv3 {
 
-constexpr int TR_TI = 32;   // interfaces per block in i
-constexpr int TR_TJ = 16;   // interfaces per block in j
+constexpr int TR_TI = 16;   // interfaces per block in i
+constexpr int TR_TJ = 24;   // interfaces per block in j
 constexpr int TR_NT = 256;  // threads per block (4 waves of 64)
 
 template <typename T>
