@@ -2,144 +2,357 @@
 //
 // Reference: fv3core/stencils/fvtp2d.py:151-362 runs eight stencils over
 // full-size HBM temporaries (y_ppm inner, q_i, x_ppm outer, x_ppm inner,
-// q_j, y_ppm outer, final_fluxes).  Here one workgroup owns a TI x TJ block
-// of interfaces on one level of one tile: it stages q (x- and
-// y-differentiable copies) with the 3-cell halo in LDS, evaluates the inner
-// sweeps, the cross-term updates and the outer sweeps in LDS, and writes only
-// the two flux fields.  HBM traffic is the compulsory reads of q, the Courant
-// numbers and the area/mass fluxes plus the two flux writes.
+// q_j, y_ppm outer, final_fluxes).  Here one kernel evaluates all of them in
+// registers (the wave-strip march below) and writes only the two flux
+// fields.  HBM traffic is the compulsory reads of q, the Courant numbers and
+// the area/mass fluxes plus the two flux writes.
 #include "common.hpp"
 #include "timing.hpp"
 #include "ppm.hpp"
 
 namespace fv3 {
 
-constexpr int TR_TI = 16;   // interfaces per block in i
-constexpr int TR_TJ = 24;   // interfaces per block in j
-constexpr int TR_NT = 256;  // threads per block (4 waves of 64)
+// Wave-strip march.  One wave (64 lanes) owns a strip of 64 columns of one
+// level of one tile: 58 output interfaces in i plus a 3-cell halo on each
+// side, lane L <-> column i0 - 3 + L.  It marches up the rows of a segment of
+// MTJ output rows.  Stencils across i read the neighbouring lanes' values
+// with DPP lane shifts; stencils along j keep a sliding window of rows in
+// registers.  Each PPM face value al and each cell's edge perturbations
+// (bl, br and the smoothness flag, or the monotone slope dm for hord 8) are
+// computed once and shared by the two interfaces that use them, with the
+// reference's expressions, so results equal the per-interface form to the
+// last bit.  No LDS and no barriers: waves run independently.
+constexpr int MW = 64;        // lanes = strip columns
+constexpr int MTI = MW - 6;   // output interfaces per strip in i
+constexpr int MTJ = 33;       // output rows per segment in j
 
+// Whole-wave shifts by one lane with DPP: wave_shr:1 (lane L reads lane
+// L - 1) and wave_shl:1 (lane L reads lane L + 1); the lane without a source
+// receives 0.  They are register moves on the VALU, without the LDS round
+// trip of a general lane permute.
+constexpr int kWaveShr1 = 0x138;
+constexpr int kWaveShl1 = 0x130;
+
+template <int CTRL>
+__device__ __forceinline__ int dpp32(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL, typename T>
+__device__ __forceinline__ T dpp_shift(T v) {
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp32<CTRL>((int)(unsigned)u);
+    const unsigned hi = (unsigned)dpp32<CTRL>((int)(unsigned)(u >> 32));
+    return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+  } else if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, dpp32<CTRL>(__builtin_bit_cast(int, v)));
+  } else {
+    return (T)dpp32<CTRL>((int)v);
+  }
+}
 template <typename T>
-__device__ __forceinline__ T ld2(const T* __restrict__ p, int i, int j, int64_t sj, int ni,
-                                 int nj) {
-  return (i >= 0 && i < ni && j >= 0 && j < nj) ? p[(int64_t)j * sj + i] : T(0);
+__device__ __forceinline__ T lane_dn(T v, int d) {  // value of lane + d
+  for (int n = 0; n < d; ++n) v = dpp_shift<kWaveShl1>(v);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T lane_up(T v, int d) {  // value of lane - d
+  for (int n = 0; n < d; ++n) v = dpp_shift<kWaveShr1>(v);
+  return v;
 }
 
 // Number of XCDs on MI355X: consecutive workgroups are dispatched round-robin
 // over them, each with its own L2.
 constexpr int kNumXcd = 8;
 
-// Workgroup id -> logical block so that each XCD runs a contiguous range of
-// logical blocks (whole planes of one level), keeping the halo rows that
-// neighbouring blocks share in that XCD's L2.
+// Workgroup id -> logical unit so that each XCD runs a contiguous range of
+// logical units (whole planes of one level), keeping the halo rows that
+// neighbouring units share in that XCD's L2.
 __device__ __forceinline__ int xcd_logical_block(int hw, int total) {
   const int per = (total + kNumXcd - 1) / kNumXcd;
   return (hw % kNumXcd) * per + hw / kNumXcd;
 }
 
+// Face value al at face f (hord < 8, compute_al, xppm.py:147-181) from the
+// cells f-2, f-1, f, f+1 (qa..qd) and their widths (da..dd, read only at the
+// tile-edge faces).  With EDGES every candidate is evaluated and the match is
+// selected without branches (the reference's regions are applied in its
+// order, so the last selection below is its first test); lanes or steps that
+// need none of them keep the interior value.
+template <typename T, bool EDGES>
+__device__ __forceinline__ T al_face(T qa, T qb, T qc, T qd, int f, int is, int ie, T da, T db,
+                                     T dc, T dd) {
+  T al = T(FV3_PPM_P1) * (qb + qc) + T(FV3_PPM_P2) * (qa + qd);
+  if constexpr (EDGES) {
+    const T c123 = T(FV3_PPM_C1) * qa + T(FV3_PPM_C2) * qb + T(FV3_PPM_C3) * qc;
+    const T ediv = T(0.5) * (((T(2) * db + da) * qb - db * qa) / (da + db) +
+                             ((T(2) * dc + dd) * qc - dc * qd) / (dc + dd));
+    const T c321 = T(FV3_PPM_C3) * qb + T(FV3_PPM_C2) * qc + T(FV3_PPM_C1) * qd;
+    al = (f == is - 1 || f == ie) ? c123 : al;
+    al = (f == is || f == ie + 1) ? ediv : al;
+    al = (f == is + 1 || f == ie + 2) ? c321 : al;
+  }
+  return al;
+}
+
+// Edge perturbations of a cell from its two face values (hord < 8) and the
+// smoothness test of hord 5 (bl br < 0) or 6/7 (3|b0| < |bl - br|).
+template <typename T, int ORD>
+__device__ __forceinline__ void cell_lo(T alL, T alR, T q, T& bl, T& br, bool& smt) {
+  bl = alL - q;
+  br = alR - q;
+  const T b0 = bl + br;
+  if constexpr (ORD == 5) {
+    smt = bl * br < T(0);
+  } else {
+    smt = (T(3) * fabs(b0)) < fabs(bl - br);
+  }
+}
+
+// Mean advected through face f from cells f-1 (m) and f (0), hord < 8.
+template <typename T>
+__device__ __forceinline__ T face_lo(T c, T qm, T blm, T brm, bool smm, T q0, T bl0, T br0,
+                                     bool sm0) {
+  const T b0m = blm + brm;
+  const T b00 = bl0 + br0;
+  const T mask = (smm || sm0) ? T(1) : T(0);
+  const T fpos = qm + (T(1) - c) * (brm - c * b0m) * mask;
+  const T fneg = q0 + (T(1) + c) * (bl0 + c * b00) * mask;
+  return c > T(0) ? fpos : fneg;
+}
+
+// hord 8 edge perturbations of cell m from cells m-2..m+2 and the slopes of
+// cells m-1, m, m+1 (compute_blbr_ord8plus away from the tile edges).
+template <typename T>
+__device__ __forceinline__ void cell8(T w1, T w2, T w3, T dmm, T dm0, T dmp, T& bl, T& br) {
+  const T third = T(1.0 / 3.0);
+  const T alL = T(0.5) * (w1 + w2) + third * (dmm - dm0);
+  const T alR = T(0.5) * (w2 + w3) + third * (dm0 - dmp);
+  const T xt = T(2) * dm0;
+  bl = -sgn(fmin(fabs(xt), fabs(alL - w2)), xt);
+  br = sgn(fmin(fabs(xt), fabs(alR - w2)), xt);
+}
+
+// Mean advected through face f, hord 8: the upwind cell's perturbations.
+template <typename T>
+__device__ __forceinline__ T face8(T c, T qm, T blm, T brm, T q0, T bl0, T br0) {
+  const bool pos = c > T(0);
+  const T bl = pos ? blm : bl0, br = pos ? brm : br0, qu = pos ? qm : q0;
+  const T b0 = bl + br;
+  const T fpos = (T(1) - c) * (br - c * b0);
+  const T fneg = (T(1) + c) * (bl + c * b0);
+  return qu + (pos ? fpos : fneg);
+}
+
+// PPM along i on one row held across the wave: lane L has cell i (= face i
+// on its left) with q, the Courant number c of face i and (tile-edge waves)
+// the width dxv of cell i; dxrow points at dx of column 0 of the row.
 template <typename T, int ORD, bool EDGES>
-__device__ __forceinline__ void fvtp2d_block(
-    const Dom& d, int i0, int j0, bool same_q, const T* __restrict__ qx,
-    const T* __restrict__ qy, const T* __restrict__ cx, const T* __restrict__ cy,
-    const T* __restrict__ ax, const T* __restrict__ ay, const T* __restrict__ ux,
-    const T* __restrict__ uy, const T* __restrict__ ar, const T* __restrict__ dx,
-    const T* __restrict__ dy, T* __restrict__ fx, T* __restrict__ fy,
-    T (*__restrict__ sqx)[TR_TI + 6], T (*__restrict__ sqy_own)[TR_TI + 6],
-    T (*__restrict__ sym)[TR_TI + 5], T (*__restrict__ sxm)[TR_TI + 1],
-    T (*__restrict__ sady)[TR_TI + 5], T (*__restrict__ sadx)[TR_TI]) {
-  constexpr int TI = TR_TI, TJ = TR_TJ;
-  constexpr int QW = TI + 6, QH = TJ + 6;
-  const int tid = threadIdx.x;
+__device__ __forceinline__ T ppm_row(T q, T c, int i, int is, int ie, T dxv,
+                                     const T* __restrict__ dxrow) {
+  const T qm1 = lane_up(q, 1), qm2 = lane_up(q, 2), qp1 = lane_dn(q, 1);
+  if constexpr (ORD < 8) {
+    T dm2 = T(0), dm1 = T(0), dp1 = T(0);
+    if constexpr (EDGES) {
+      dm2 = lane_up(dxv, 2);
+      dm1 = lane_up(dxv, 1);
+      dp1 = lane_dn(dxv, 1);
+    }
+    const T al = al_face<T, EDGES>(qm2, qm1, q, qp1, i, is, ie, dm2, dm1, dxv, dp1);
+    const T alp = lane_dn(al, 1);
+    T bl, br;
+    bool sm;
+    cell_lo<T, ORD>(al, alp, q, bl, br, sm);
+    const T blm = lane_up(bl, 1), brm = lane_up(br, 1);
+    const bool smm = lane_up((int)sm, 1) != 0;
+    return face_lo(c, qm1, blm, brm, smm, q, bl, br, sm);
+  } else {
+    const T qp2 = lane_dn(q, 2);
+    T bl, br;
+    if constexpr (EDGES) {
+      const T w[5] = {qm2, qm1, q, qp1, qp2};
+      DxRow<T> dw{dxrow + (i - 2), 1, 5};
+      ppm_blbr8_w(w, i, is, ie, dw, bl, br);
+    } else {
+      const T dm = ppm_dm(qm1, q, qp1);
+      cell8(qm1, q, qp1, lane_up(dm, 1), dm, lane_dn(dm, 1), bl, br);
+    }
+    return face8(c, qm1, lane_up(bl, 1), lane_up(br, 1), q, bl, br);
+  }
+}
+
+// PPM along j in one column: cells are pushed bottom to top; after cell c is
+// pushed, face c - 2 (between cells c - 3 and c - 2) can be evaluated.
+template <typename T, int ORD, bool EDGES>
+struct ColumnPPM {
+  T q[5] = {};    // cells c-4 .. c
+  T dw[5] = {};   // their widths (tile-edge waves, hord < 8)
+  T al1 = 0, al2 = 0;            // faces c-2, c-1 (hord < 8)
+  T dm0 = 0, dm1 = 0, dm2 = 0;   // slopes of cells c-3, c-2, c-1 (hord 8)
+  T bl0 = 0, br0 = 0, bl1 = 0, br1 = 0;  // cells c-3, c-2
+  bool sm0 = false, sm1 = false;
+
+  __device__ __forceinline__ void push(T qc, T dc, int c, int is, int ie,
+                                       const T* __restrict__ dycol, int64_t sj2) {
+#pragma unroll
+    for (int n = 0; n < 4; ++n) q[n] = q[n + 1];
+    q[4] = qc;
+    bl0 = bl1;
+    br0 = br1;
+    sm0 = sm1;
+    if constexpr (ORD < 8) {
+      if constexpr (EDGES) {
+#pragma unroll
+        for (int n = 0; n < 4; ++n) dw[n] = dw[n + 1];
+        dw[4] = dc;
+      }
+      al1 = al2;
+      al2 = al_face<T, EDGES>(q[1], q[2], q[3], q[4], c - 1, is, ie, dw[1], dw[2], dw[3], dw[4]);
+      cell_lo<T, ORD>(al1, al2, q[2], bl1, br1, sm1);
+    } else {
+      dm0 = dm1;
+      dm1 = dm2;
+      dm2 = ppm_dm(q[2], q[3], q[4]);
+      if constexpr (EDGES) {
+        const T w[5] = {q[0], q[1], q[2], q[3], q[4]};
+        DxRow<T> dwr{dycol + (int64_t)(c - 4) * sj2, sj2, 5};
+        ppm_blbr8_w(w, c - 2, is, ie, dwr, bl1, br1);
+      } else {
+        cell8(q[1], q[2], q[3], dm0, dm1, dm2, bl1, br1);
+      }
+    }
+  }
+  // mean advected through face c - 2
+  __device__ __forceinline__ T face(T cc) const {
+    if constexpr (ORD < 8) {
+      return face_lo(cc, q[1], bl0, br0, sm0, q[2], bl1, br1, sm1);
+    } else {
+      return face8(cc, q[1], bl0, br0, q[2], bl1, br1);
+    }
+  }
+};
+
+template <typename T, int ORD, bool EX, bool EY>
+__device__ __forceinline__ void fvtp2d_strip(
+    const Dom& d, int i0, int j0, const T* __restrict__ qx, const T* __restrict__ qy,
+    const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ ax,
+    const T* __restrict__ ay, const T* __restrict__ ux, const T* __restrict__ uy,
+    const T* __restrict__ ar, const T* __restrict__ dx, const T* __restrict__ dy,
+    T* __restrict__ fx, T* __restrict__ fy) {
+  const int lane = threadIdx.x;
+  const int i = i0 - 3 + lane;  // column of this lane
   const int ni = d.ni, nj = d.nj;
-  const int sj = (int)d.sj, sj2 = (int)d.sj2;
-  T (*sqy)[QW] = same_q ? sqx : sqy_own;
-  auto ld = [&](const T* __restrict__ p, int i, int j, int stride) -> T {
-    return (i >= 0 && i < ni && j >= 0 && j < nj) ? p[j * stride + i] : T(0);
+  const int64_t sj = d.sj, sj2 = d.sj2;
+  const bool out_lane = lane >= 3 && lane < 3 + MTI;
+  // Loads read rows and columns clamped into the storage, unconditionally.
+  // A lane whose column lies outside the storage, or a step whose row does,
+  // feeds only interfaces beyond the compute domain (every flux kept reads
+  // cells inside the storage), so what such loads return is never used.
+  const int ic = min(max(i, 0), ni - 1);
+  const uint32_t icb = (uint32_t)ic * (uint32_t)sizeof(T);  // byte offset of the column
+  // The column offset is passed through an empty asm at each use so that the
+  // compiler keeps (uniform row address) + (32-bit lane offset), which maps
+  // to the scalar-base form of the global loads and stores, instead of
+  // hoisting 64-bit per-lane pointers and adding the row in vector registers.
+  auto lane_off = [&]() {
+    uint32_t o = icb;
+    asm volatile("" : "+v"(o));
+    return o;
   };
-
-  // 1. stage q_x (and q_y unless it is the same storage) with a 3-cell halo
-  for (int n = tid; n < QW * QH; n += TR_NT) {
-    const int li = n % QW, lj = n / QW;
-    const int i = i0 - 3 + li, j = j0 - 3 + lj;
-    sqx[lj][li] = ld(qx, i, j, sj);
-    if (!same_q) sqy_own[lj][li] = ld(qy, i, j, sj);
-  }
-  __syncthreads();
-
-  // 2a. x-inner PPM on q_x: faces i in [i0, i0+TI], rows j in [j0-3, j0+TJ+2)
-  for (int n = tid; n < (TI + 1) * (TJ + 5); n += TR_NT) {
-    const int li = n % (TI + 1), lj = n / (TI + 1);
-    const int i = i0 + li, j = j0 - 3 + lj;
-    T q6[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) q6[m] = sqx[lj][li + m];
-    const T c = ld(cx, i, j, sj);
-    DxRow<T> dr{dx + (int64_t)j * sj2 + (i - 3), 1};
-    sxm[lj][li] = (j < nj) ? ppm_face_sel<T, ORD, EDGES>(q6, c, i, d.is, d.ie, dr) : T(0);
-  }
-  // 2b. y-inner PPM on q_y: faces j in [j0, j0+TJ], columns i in [i0-3, i0+TI+2)
-  for (int n = tid; n < (TI + 5) * (TJ + 1); n += TR_NT) {
-    const int li = n % (TI + 5), lj = n / (TI + 5);
-    const int i = i0 - 3 + li, j = j0 + lj;
-    T q6[6];
-#pragma unroll
-    for (int m = 0; m < 6; ++m) q6[m] = sqy[lj + m][li];
-    const T c = ld(cy, i, j, sj);
-    DxRow<T> dr{dy + (int64_t)(j - 3) * sj2 + i, sj2};
-    sym[lj][li] = (i < ni) ? ppm_face_sel<T, ORD, EDGES>(q6, c, j, d.js, d.je, dr) : T(0);
-  }
-  __syncthreads();
-
-  // 3a. q_j (fvtp2d.py:50-60): cells i in [i0, i0+TI), rows [j0-3, j0+TJ+2)
-  for (int n = tid; n < TI * (TJ + 5); n += TR_NT) {
-    const int li = n % TI, lj = n / TI;
-    const int i = i0 + li, j = j0 - 3 + lj;
-    const T a = ld(ar, i, j, sj2);
-    const T f0 = ld(ax, i, j, sj);
-    const T f1 = ld(ax, i + 1, j, sj);
-    const T fx1_0 = f0 * sxm[lj][li];
-    const T fx1_1 = f1 * sxm[lj][li + 1];
-    const T area_with_x_flux = a + f0 - f1;
-    sadx[lj][li] = (sqx[lj][li + 3] * a + fx1_0 - fx1_1) / area_with_x_flux;
-  }
-  // 3b. q_i (fvtp2d.py:33-47): cells i in [i0-3, i0+TI+2), rows [j0, j0+TJ)
-  for (int n = tid; n < (TI + 5) * TJ; n += TR_NT) {
-    const int li = n % (TI + 5), lj = n / (TI + 5);
-    const int i = i0 - 3 + li, j = j0 + lj;
-    const T a = ld(ar, i, j, sj2);
-    const T g0 = ld(ay, i, j, sj);
-    const T g1 = ld(ay, i, j + 1, sj);
-    const T fyy0 = g0 * sym[lj][li];
-    const T fyy1 = g1 * sym[lj + 1][li];
-    sady[lj][li] = (sqy[lj + 3][li] * a + fyy0 - fyy1) / (a + g0 - g1);
-  }
-  __syncthreads();
-
-  // 4. outer sweeps and final_fluxes (fvtp2d.py:63-92)
-  for (int n = tid; n < TI * TJ; n += TR_NT) {
-    const int li = n % TI, lj = n / TI;
-    const int i = i0 + li, j = j0 + lj;
-    const int o = j * sj + i;
-    if (i <= d.iec + 1 && j <= d.jec) {
-      T q6[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) q6[m] = sady[lj][li + m];
-      DxRow<T> dr{dx + (int64_t)j * sj2 + (i - 3), 1};
-      const T outer = ppm_face_sel<T, ORD, EDGES>(q6, cx[o], i, d.is, d.ie, dr);
-      fx[o] = T(0.5) * (outer + sxm[lj + 3][li]) * ux[o];
+  auto ld = [&](const T* __restrict__ p, int j, int64_t stride) -> T {
+    const int jc = __builtin_amdgcn_readfirstlane(min(max(j, 0), nj - 1));
+    const char* __restrict__ row = (const char*)(p + (int64_t)jc * stride);  // wave-uniform
+    return *(const T*)(row + lane_off());
+  };
+  // Flux stores are buffer stores: lanes or steps that own no output give
+  // an offset beyond the buffer's record range, which the hardware drops,
+  // so the march body has no branches around its stores.
+  const __amdgpu_buffer_rsrc_t rfx = __builtin_amdgcn_make_buffer_rsrc(fx, 0, 0x7fffffff, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rfy = __builtin_amdgcn_make_buffer_rsrc(fy, 0, 0x7fffffff, 0x00020000);
+  auto st = [&](__amdgpu_buffer_rsrc_t rs, int j, bool keep, T v) {
+    const uint32_t off = keep ? icb : 0x80000000u;
+    const int rowb = __builtin_amdgcn_readfirstlane((int)((int64_t)min(max(j, 0), nj - 1) * sj *
+                                                          (int64_t)sizeof(T)));
+    if constexpr (sizeof(T) == 8) {
+      typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, off, rowb, 0);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off, rowb, 0);
     }
-    if (i <= d.iec && j <= d.jec + 1) {
-      T q6[6];
+  };
+  ColumnPPM<T, ORD, EY> yin, yout;
+  T sxm[4] = {};     // x-inner means of rows r-3 .. r at face i
+  T sym_prev = T(0); // y-inner mean of face r-3
+  // tile-edge width rows are clamped into the storage the same way
+  auto jrow = [&](int j) { return (int64_t)min(max(j, 0), nj - 1) * sj2; };
+  const T* dycol = dy + ic;
+  // every value a march step uses is loaded by the step before it, so the
+  // loads of one step are in flight while the previous step computes
+  struct StepIn {
+    T qx, qy, cx, ax, ar, dx, dy;  // row r
+    T cyf, ayf;                    // row f = r - 2
+    T ag, cxg, dxg;                // row g = r - 3
+    T uxg, uyf;                    // the mass fluxes at the outputs
+  };
+  auto load_step = [&](int r, StepIn& v) {
+    const int f = r - 2, g = r - 3;
+    v.qx = ld(qx, r, sj);
+    v.qy = ld(qy, r, sj);
+    v.cx = ld(cx, r, sj);
+    v.ax = ld(ax, r, sj);
+    v.ar = ld(ar, r, sj2);
+    v.dx = EX ? ld(dx, r, sj2) : T(0);
+    v.dy = EY ? ld(dy, r, sj2) : T(0);
+    v.cyf = ld(cy, f, sj);
+    v.ayf = ld(ay, f, sj);
+    v.ag = ld(ar, g, sj2);
+    v.cxg = ld(cx, g, sj);
+    v.dxg = EX ? ld(dx, g, sj2) : T(0);
+    v.uxg = ld(ux, g, sj);
+    v.uyf = ld(uy, f, sj);
+  };
+  StepIn nx;
+  load_step(j0 - 3, nx);
+  T ay_g = T(0);  // ay of row g = r - 3 (row f of the previous step)
+#pragma unroll 3
+  for (int s = 0; s < MTJ + 6; ++s) {
+    const int r = j0 - 3 + s;
+    const StepIn cu = nx;
+    load_step(r + 1, nx);
+    const int f = r - 2, g = r - 3;
+    const bool wfx = out_lane && g >= j0 && g < j0 + MTJ && i <= d.iec + 1 && g <= d.jec;
+    const bool wfy = out_lane && f >= j0 && f < j0 + MTJ && i <= d.iec && f <= d.jec + 1;
+    // x-inner PPM of q_x on row r (faces i), then q_j of row r
+    // (fvtp2d.py:50-60)
+    const T xm = ppm_row<T, ORD, EX>(cu.qx, cu.cx, i, d.is, d.ie, cu.dx, dx + jrow(r));
+    const T sxr = xm;
+    const T f1 = lane_dn(cu.ax, 1);
+    const T fx1_0 = cu.ax * sxr;
+    const T fx1_1 = f1 * lane_dn(sxr, 1);
+    const T qjr = (cu.qx * cu.ar + fx1_0 - fx1_1) / (cu.ar + cu.ax - f1);
 #pragma unroll
-      for (int m = 0; m < 6; ++m) q6[m] = sadx[lj + m][li];
-      DxRow<T> dr{dy + (int64_t)(j - 3) * sj2 + i, sj2};
-      const T outer = ppm_face_sel<T, ORD, EDGES>(q6, cy[o], j, d.js, d.je, dr);
-      fy[o] = T(0.5) * (outer + sym[lj][li + 3]) * uy[o];
-    }
+    for (int n = 0; n < 3; ++n) sxm[n] = sxm[n + 1];
+    sxm[3] = sxr;
+    // y-inner PPM of q_y: face f = r - 2
+    yin.push(cu.qy, cu.dy, r, d.js, d.je, dycol, sj2);
+    const T symf = yin.face(cu.cyf);
+    // q_i of row g = r - 3 (fvtp2d.py:33-47)
+    const T qig = (yin.q[1] * cu.ag + ay_g * sym_prev - cu.ayf * symf) /
+                  (cu.ag + ay_g - cu.ayf);
+    sym_prev = symf;
+    ay_g = cu.ayf;
+    // x-outer on q_i of row g: q_x_flux (fvtp2d.py:63-92)
+    const T outx = ppm_row<T, ORD, EX>(qig, cu.cxg, i, d.is, d.ie, cu.dxg, dx + jrow(g));
+    st(rfx, g, wfx, T(0.5) * (outx + sxm[0]) * cu.uxg);
+    // y-outer PPM of q_j: face f, q_y_flux
+    yout.push(qjr, cu.dy, r, d.js, d.je, dycol, sj2);
+    const T outy = yout.face(cu.cyf);
+    st(rfy, f, wfy, T(0.5) * (outy + symf) * cu.uyf);
   }
 }
 
 template <typename T, int ORD>
-__global__ void __launch_bounds__(TR_NT)
+__global__ void __launch_bounds__(MW)
     fvtp2d_kernel(Dom d, const T* __restrict__ q_x, const T* __restrict__ q_y,
                   const T* __restrict__ crx, const T* __restrict__ cry,
                   const T* __restrict__ xfx, const T* __restrict__ yfx,
@@ -147,15 +360,6 @@ __global__ void __launch_bounds__(TR_NT)
                   const T* __restrict__ area, const T* __restrict__ dxa,
                   const T* __restrict__ dya, T* __restrict__ fx, T* __restrict__ fy,
                   int nbi, int nbj) {
-  constexpr int TI = TR_TI, TJ = TR_TJ;
-  constexpr int QW = TI + 6, QH = TJ + 6;
-  __shared__ T sqx[QH][QW];
-  __shared__ T sqy[QH][QW];
-  __shared__ T sym[TJ + 1][TI + 5];   // y-inner means, faces (i0-3.., j0..)
-  __shared__ T sxm[TJ + 5][TI + 1];   // x-inner means, faces (i0.., j0-3..)
-  __shared__ T sady[TJ][TI + 5];      // q advected in y, cells (i0-3.., j0..)
-  __shared__ T sadx[TJ + 5][TI];      // q advected in x, cells (i0.., j0-3..)
-
   const int per_plane = nbi * nbj;
   const int total = per_plane * d.nk * d.ntiles;
   const int lb = xcd_logical_block(blockIdx.x, total);
@@ -164,36 +368,32 @@ __global__ void __launch_bounds__(TR_NT)
   const int within = lb % per_plane;
   const int bi = within % nbi, bj = within / nbi;
   const int k = plane % d.nk, t = plane / d.nk;
-  const int i0 = d.isc + bi * TI;
-  const int j0 = d.jsc + bj * TJ;
+  const int i0 = d.isc + bi * MTI;
+  const int j0 = d.jsc + bj * MTJ;
   const int64_t off3 = (int64_t)t * d.st + (int64_t)k * d.sk;
   const int64_t off2 = (int64_t)t * d.st2;
-  const bool same_q = (q_x == q_y);
-  const T* qx = q_x + off3;
-  const T* qy = q_y + off3;
-  const T* cx = crx + off3;
-  const T* cy = cry + off3;
-  const T* ax = xfx + off3;
-  const T* ay = yfx + off3;
   const T* ux = (xmf ? xmf : xfx) + off3;
   const T* uy = (ymf ? ymf : yfx) + off3;
-  const T* ar = area + off2;
-  const T* dx = dxa + off2;
-  const T* dy = dya + off2;
-  T* fxp = fx + off3;
-  T* fyp = fy + off3;
-  // a block needs the tile-edge formulas when its stencil reach (cells
-  // i0-4 .. i0+TI+4) touches the edge regions (anchor -2 .. +3)
+  // the tile-edge formulas of a direction are needed when the strip's
+  // stencil reach (cells i0-4 .. i0+MTI+4, rows j0-4 .. j0+MTJ+4) touches
+  // an edge region of that direction
   auto near = [](int lo, int hi, int a) { return hi >= a - 2 && lo <= a + 3; };
-  const bool edges = near(i0 - 4, i0 + TI + 4, d.is) || near(i0 - 4, i0 + TI + 4, d.ie) ||
-                     near(j0 - 4, j0 + TJ + 4, d.js) || near(j0 - 4, j0 + TJ + 4, d.je);
-  if (edges) {
-    fvtp2d_block<T, ORD, true>(d, i0, j0, same_q, qx, qy, cx, cy, ax, ay, ux, uy, ar, dx, dy,
-                               fxp, fyp, sqx, sqy, sym, sxm, sady, sadx);
+  const bool ex = near(i0 - 4, i0 + MTI + 4, d.is) || near(i0 - 4, i0 + MTI + 4, d.ie);
+  const bool ey = near(j0 - 4, j0 + MTJ + 4, d.js) || near(j0 - 4, j0 + MTJ + 4, d.je);
+#define FV3_TP_STRIP(EX_, EY_)                                                                  \
+  fvtp2d_strip<T, ORD, EX_, EY_>(d, i0, j0, q_x + off3, q_y + off3, crx + off3, cry + off3,   \
+                                 xfx + off3, yfx + off3, ux, uy, area + off2, dxa + off2,     \
+                                 dya + off2, fx + off3, fy + off3)
+  if (ex && ey) {
+    FV3_TP_STRIP(true, true);
+  } else if (ex) {
+    FV3_TP_STRIP(true, false);
+  } else if (ey) {
+    FV3_TP_STRIP(false, true);
   } else {
-    fvtp2d_block<T, ORD, false>(d, i0, j0, same_q, qx, qy, cx, cy, ax, ay, ux, uy, ar, dx, dy,
-                                fxp, fyp, sqx, sqy, sym, sxm, sady, sadx);
+    FV3_TP_STRIP(false, false);
   }
+#undef FV3_TP_STRIP
 }
 
 template <typename T>
@@ -211,13 +411,16 @@ static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T*
   if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
   FV3_REQUIRE(dom->sj < (1 << 30) && dom->sj2 < (1 << 30), FV3_E_ARG, "%s: row stride too large",
               who);
+  // flux stores address a level by a 32-bit byte offset
+  FV3_REQUIRE((int64_t)dom->nj * dom->sj * (int64_t)sizeof(T) < ((int64_t)1 << 31), FV3_E_ARG,
+              "%s: level too large", who);
   Dom d = to_dom(dom);
   const int nfi = d.iec + 2 - d.isc, nfj = d.jec + 2 - d.jsc;
-  const int nbi = (nfi + TR_TI - 1) / TR_TI, nbj = (nfj + TR_TJ - 1) / TR_TJ;
+  const int nbi = (nfi + MTI - 1) / MTI, nbj = (nfj + MTJ - 1) / MTJ;
   const int64_t total = (int64_t)nbi * nbj * d.nk * d.ntiles;
   FV3_REQUIRE(total < (int64_t)1 << 30, FV3_E_ARG, "%s: grid too large", who);
   const int64_t launched = (total + kNumXcd - 1) / kNumXcd * kNumXcd;
-  dim3 grid((unsigned)launched), block(TR_NT);
+  dim3 grid((unsigned)launched), block(MW);
   hipStream_t s = as_stream(stream);
   // algorithmic bytes: q (and its y-corner copy when it is a separate
   // array), crx, cry, both area fluxes and the optional mass fluxes read
