@@ -57,6 +57,7 @@ _SIGNATURES = {
     "fv3_version": ([], ctypes.c_char_p),
     "fv3_last_error": ([], ctypes.c_char_p),
     "fv3_device_count": ([], _I),
+    "fv3_fvtp2d_workspace_slabs": ([], _I),
 }
 
 

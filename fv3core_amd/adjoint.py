@@ -1,0 +1,226 @@
+"""Adjoint of PPM transport (fv_tp_2d) behind ``torch.autograd``.
+
+``FvTp2dFn`` is a :class:`torch.autograd.Function` whose forward runs the
+fused transport kernel ``fv3_fvtp2d`` and whose backward runs the
+hand-written adjoint ``fv3_fvtp2d_adjoint``.  The adjoint re-linearises the
+forward at the saved input (every hord limiter branch, smoothness mask and
+upwind choice is taken as the forward took it), so each call keeps only its
+input state: the call is its own checkpoint.  ``TransportLinearization``
+also exposes the tangent-linear model (``fv3_fvtp2d_tlm``) for dot-product
+tests.
+
+``PeriodicTransport`` steps a scalar on a doubly periodic tile with the
+flux-form update ``q += (fx(i) - fx(i+1) + fy(j) - fy(j+1)) / area`` (the
+``q_adjust`` form of fv3core/stencils/tracer_2d_1l.py:108-117 with unit
+layer mass), and :func:`cost_and_gradient` evaluates the 4D-Var-style cost
+``J = 0.5 ||q_T - q_obs||^2`` over ``n`` steps and ``dJ/dq0`` by the reverse
+sweep.  Courant numbers, area fluxes and metric terms are held fixed.
+
+Tensors handed to the kernels are i-fastest ``(I, J, K)`` storages.  The
+autograd-side state is kept as row-major ``(K, J, I)`` tensors, whose
+``permute(2, 1, 0)`` view is exactly that layout.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import torch
+import torch.nn.functional as F
+
+from . import _lib
+
+
+def ijk_view(kji: torch.Tensor) -> torch.Tensor:
+    """(I, J, K) i-fastest view of a row-major (K, J, I) tensor."""
+    return kji.permute(2, 1, 0)
+
+
+def as_ijk_storage(t: torch.Tensor) -> torch.Tensor:
+    """``t`` (I, J, K) with i-fastest strides (copied if it has other strides)."""
+    if t.dim() == 3 and t.stride(0) == 1 and t.stride(1) == t.shape[0] and \
+            t.stride(2) == t.shape[0] * t.shape[1]:
+        return t
+    return ijk_view(t.permute(2, 1, 0).contiguous())
+
+
+def new_ijk(like: torch.Tensor) -> torch.Tensor:
+    """Zeroed (I, J, K) i-fastest storage shaped like ``like``."""
+    ni, nj, nk = like.shape
+    return ijk_view(torch.zeros((nk, nj, ni), dtype=like.dtype, device=like.device))
+
+
+class TransportLinearization:
+    """fv_tp_2d forward, tangent-linear and adjoint on one (I, J, K) storage.
+
+    ``grid_indexing`` gives the compute domain and tile-edge anchors (as for
+    FiniteVolumeTransport); ``area``, ``dxa``, ``dya`` are IJ metric fields;
+    ``hord`` is 5, 6, 7 or 8; ``nk`` the number of levels transported.
+    """
+
+    def __init__(self, grid_indexing, area, dxa, dya, hord: int, nk: int):
+        if hord not in (5, 6, 7, 8):
+            raise NotImplementedError(f"hord={hord} is not supported (5, 6, 7, 8)")
+        _lib.require_device(area, dxa, dya)
+        if area.dtype != torch.float64:
+            raise TypeError("the transport adjoint computes in float64")
+        self.gi = grid_indexing
+        self.area, self.dxa, self.dya = area, dxa, dya
+        self.hord = int(hord)
+        self.nk = int(nk)
+        self._work = None
+        self._slab = 0
+
+    def _domain(self, qx, *others):
+        return _lib.make_domain(self.gi, qx, self.nk, self.area, others3=others,
+                                others2=(self.dxa, self.dya))
+
+    def _workspace(self, ref):
+        ntiles, ni, nj, _, sj, sk, st = _lib.layout3(ref)
+        need = (ntiles - 1) * st + self.nk * sk
+        need = (need + 63) // 64 * 64
+        nslabs = _lib.lib().fv3_fvtp2d_workspace_slabs()
+        if self._work is None or self._slab < need or self._work.device != ref.device:
+            self._work = torch.empty(nslabs * need, dtype=torch.float64, device=ref.device)
+            self._slab = need
+        return self._work, self._slab
+
+    def forward(self, qx, qy, crx, cry, xfx, yfx, xmf=None, ymf=None):
+        """(fx, fy) = fv_tp_2d(q) on fresh zeroed storages."""
+        _lib.require_device(qx, qy, crx, cry, xfx, yfx, xmf, ymf)
+        fx, fy = new_ijk(qx), new_ijk(qx)
+        d = self._domain(qx, qy, crx, cry, xfx, yfx, xmf, ymf, fx, fy)
+        _lib.check(_lib.lib().fv3_fvtp2d_f64(
+            ctypes.byref(d), self.hord, _lib.ptr(qx), _lib.ptr(qy), _lib.ptr(crx),
+            _lib.ptr(cry), _lib.ptr(xfx), _lib.ptr(yfx), _lib.ptr(xmf), _lib.ptr(ymf),
+            _lib.ptr(self.area), _lib.ptr(self.dxa), _lib.ptr(self.dya), _lib.ptr(fx),
+            _lib.ptr(fy), ctypes.c_void_p(_lib.stream_handle())))
+        return fx, fy
+
+    def tlm(self, qx, qy, crx, cry, xfx, yfx, dqx, dqy, xmf=None, ymf=None):
+        """(dfx, dfy): the transport linearised at (qx, qy) applied to (dqx, dqy)."""
+        _lib.require_device(qx, qy, crx, cry, xfx, yfx, xmf, ymf, dqx, dqy)
+        dfx, dfy = new_ijk(qx), new_ijk(qx)
+        d = self._domain(qx, qy, crx, cry, xfx, yfx, xmf, ymf, dqx, dqy, dfx, dfy)
+        work, slab = self._workspace(qx)
+        _lib.check(_lib.lib().fv3_fvtp2d_tlm_f64(
+            ctypes.byref(d), self.hord, _lib.ptr(qx), _lib.ptr(qy), _lib.ptr(crx),
+            _lib.ptr(cry), _lib.ptr(xfx), _lib.ptr(yfx), _lib.ptr(xmf), _lib.ptr(ymf),
+            _lib.ptr(self.area), _lib.ptr(self.dxa), _lib.ptr(self.dya), _lib.ptr(dqx),
+            _lib.ptr(dqy), _lib.ptr(dfx), _lib.ptr(dfy), _lib.ptr(work), ctypes.c_int64(slab),
+            ctypes.c_void_p(_lib.stream_handle())))
+        return dfx, dfy
+
+    def adjoint(self, qx, qy, crx, cry, xfx, yfx, gfx, gfy, xmf=None, ymf=None):
+        """(g_qx, g_qy): the transposed linearisation applied to (gfx, gfy)."""
+        _lib.require_device(qx, qy, crx, cry, xfx, yfx, xmf, ymf, gfx, gfy)
+        gqx, gqy = new_ijk(qx), new_ijk(qx)
+        d = self._domain(qx, qy, crx, cry, xfx, yfx, xmf, ymf, gfx, gfy, gqx, gqy)
+        work, slab = self._workspace(qx)
+        _lib.check(_lib.lib().fv3_fvtp2d_adjoint_f64(
+            ctypes.byref(d), self.hord, _lib.ptr(qx), _lib.ptr(qy), _lib.ptr(crx),
+            _lib.ptr(cry), _lib.ptr(xfx), _lib.ptr(yfx), _lib.ptr(xmf), _lib.ptr(ymf),
+            _lib.ptr(self.area), _lib.ptr(self.dxa), _lib.ptr(self.dya), _lib.ptr(gfx),
+            _lib.ptr(gfy), _lib.ptr(gqx), _lib.ptr(gqy), _lib.ptr(work), ctypes.c_int64(slab),
+            ctypes.c_void_p(_lib.stream_handle())))
+        return gqx, gqy
+
+
+class FvTp2dFn(torch.autograd.Function):
+    """(fx, fy) = fv_tp_2d(q_x, q_y) with the hand-written adjoint as backward.
+
+    ``aux`` = (crx, cry, x_area_flux, y_area_flux, x_mass_flux, y_mass_flux)
+    is held fixed (no gradient flows to it).
+    """
+
+    @staticmethod
+    def forward(ctx, qx, qy, lin: TransportLinearization, aux):
+        qx_s, qy_s = as_ijk_storage(qx), as_ijk_storage(qy)
+        fx, fy = lin.forward(qx_s, qy_s, *aux[:4], aux[4], aux[5])
+        ctx.save_for_backward(qx_s, qy_s)
+        ctx.lin = lin
+        ctx.aux = aux
+        return fx, fy
+
+    @staticmethod
+    def backward(ctx, gfx, gfy):
+        qx, qy = ctx.saved_tensors
+        aux = ctx.aux
+        gfx = torch.zeros_like(qx) if gfx is None else gfx
+        gfy = torch.zeros_like(qx) if gfy is None else gfy
+        gqx, gqy = ctx.lin.adjoint(qx, qy, *aux[:4], as_ijk_storage(gfx),
+                                   as_ijk_storage(gfy), aux[4], aux[5])
+        return gqx, gqy, None, None
+
+
+class PeriodicTransport:
+    """Flux-form PPM transport of a scalar on one doubly periodic tile.
+
+    ``inputs`` holds device (I, J, K) storages ``crx``, ``cry``,
+    ``x_area_flux``, ``y_area_flux`` and IJ fields ``area``, ``dxa``, ``dya``
+    (as :func:`fv3core_amd.workloads.transport_inputs` builds them).  The
+    state is a row-major (K, J, I) tensor of the storage shape; only its
+    compute interior is meaningful, the halo is refilled by wrapping before
+    every step.
+    """
+
+    def __init__(self, grid_indexing, inputs: dict, hord: int, n_halo: int = 3):
+        gi = grid_indexing
+        self.gi = gi
+        self.h = n_halo
+        self.n = gi.domain[0]
+        if gi.domain[0] != gi.domain[1]:
+            raise ValueError("a square tile is expected")
+        crx = as_ijk_storage(inputs["crx"])
+        ni, nj, nk = crx.shape
+        self.shape = (ni, nj, nk)
+        self.lin = TransportLinearization(gi, inputs["area"], inputs["dxa"], inputs["dya"],
+                                          hord, nk)
+        self.aux = (crx, as_ijk_storage(inputs["cry"]), as_ijk_storage(inputs["x_area_flux"]),
+                    as_ijk_storage(inputs["y_area_flux"]), None, None)
+        dev = crx.device
+        h, n = self.h, self.n
+        self._ii = (h + (torch.arange(ni, device=dev) - h) % n)
+        self._jj = (h + (torch.arange(nj, device=dev) - h) % n)
+        rarea = 1.0 / inputs["area"]                       # (I, J)
+        self._rarea = rarea.t()[None, h:h + n, h:h + n]   # (1, J, I) interior
+
+    def fill(self, q_kji: torch.Tensor) -> torch.Tensor:
+        """Halo of a (K, J, I) state refilled by wrapping (differentiable)."""
+        return q_kji[:, self._jj][:, :, self._ii]
+
+    def step(self, q_kji: torch.Tensor) -> torch.Tensor:
+        h, n = self.h, self.n
+        qf = self.fill(q_kji)
+        q = ijk_view(qf)
+        fx, fy = FvTp2dFn.apply(q, q, self.lin, self.aux)
+        fx_kji = fx.permute(2, 1, 0)
+        fy_kji = fy.permute(2, 1, 0)
+        c = slice(h, h + n)
+        c1 = slice(h + 1, h + n + 1)
+        dq = (fx_kji[:, c, c] - fx_kji[:, c, c1] + fy_kji[:, c, c] - fy_kji[:, c1, c]) * \
+            self._rarea
+        ni, nj, _ = self.shape
+        dq = F.pad(dq, (h, ni - h - n, h, nj - h - n))
+        return qf + dq
+
+    def run(self, q0_kji: torch.Tensor, nsteps: int) -> torch.Tensor:
+        q = q0_kji
+        for _ in range(nsteps):
+            q = self.step(q)
+        return q
+
+    def interior(self, q_kji: torch.Tensor) -> torch.Tensor:
+        h, n = self.h, self.n
+        return q_kji[:, h:h + n, h:h + n]
+
+
+def cost_and_gradient(model: PeriodicTransport, q0_kji: torch.Tensor, q_obs_kji: torch.Tensor,
+                      nsteps: int) -> Tuple[float, torch.Tensor]:
+    """J = 0.5 ||q_T - q_obs||^2 over the compute interior, and dJ/dq0."""
+    q0 = q0_kji.detach().clone().requires_grad_(True)
+    qt = model.run(q0, nsteps)
+    r = model.interior(qt) - model.interior(q_obs_kji)
+    cost = 0.5 * (r * r).sum()
+    (grad,) = torch.autograd.grad(cost, q0)
+    return float(cost.detach()), grad

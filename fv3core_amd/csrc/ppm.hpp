@@ -76,24 +76,25 @@ __device__ __forceinline__ T ppm_edge1(const T* q, const T* dx, int p) {
 }
 
 // Loads dx[i-3..i+2] of the interface's row/column only when an edge formula
-// needs it.
-template <typename T>
+// needs it.  S is the stored type of the widths, T the type the PPM
+// arithmetic runs in (the same, or a derivative-carrying type built from S).
+template <typename T, typename S = T>
 struct DxRow {
-  const T* base;  // points at dx of cell i-3
+  const S* base;  // points at dx of cell i-3
   int64_t stride;
   int count = 6;  // entries that may be read (the rest are filled with 1)
   __device__ __forceinline__ void load(T (&d)[6]) const {
 #pragma unroll
-    for (int n = 0; n < 6; ++n) d[n] = n < count ? base[n * stride] : T(1);
+    for (int n = 0; n < 6; ++n) d[n] = n < count ? T(base[n * stride]) : T(1);
   }
 };
 
 // Edge perturbations bl, br of cell m = absolute index, located at q[p]
 // (hord = 8: compute_blbr_ord8plus + bl_br_edges, xppm.py:92-102, 184-266).
 // q[p-2..p+2] and dx[p-2..p+2] must be valid.
-template <typename T>
+template <typename T, typename S>
 __device__ __forceinline__ void ppm_blbr8(const T* q, int p, int m, int is, int ie,
-                                          const DxRow<T>& dxr, T& bl, T& br) {
+                                          const DxRow<T, S>& dxr, T& bl, T& br) {
   const T dmm = ppm_dm(q[p - 2], q[p - 1], q[p]);
   const T dm0 = ppm_dm(q[p - 1], q[p], q[p + 1]);
   const T dmp = ppm_dm(q[p], q[p + 1], q[p + 2]);
@@ -142,9 +143,9 @@ __device__ __forceinline__ void ppm_blbr8(const T* q, int p, int m, int is, int 
 
 // ppm_blbr8 on a 5-cell window w[0..4] centred on the cell (w[2]); dxr points
 // at dx of the window's first cell.
-template <typename T>
+template <typename T, typename S>
 __device__ __forceinline__ void ppm_blbr8_w(const T (&w)[5], int m, int is, int ie,
-                                            const DxRow<T>& dxr, T& bl, T& br) {
+                                            const DxRow<T, S>& dxr, T& bl, T& br) {
   T q[6] = {w[0], w[1], w[2], w[3], w[4], w[4]};
   // ppm_blbr8 reads q[p-2..p+2] and loads dx[0..5] relative to its own
   // origin; with p = 2 the window's first cell is the origin
@@ -153,9 +154,9 @@ __device__ __forceinline__ void ppm_blbr8_w(const T (&w)[5], int m, int is, int 
 
 // Interface value al at face f whose left cell is q[p-1] (hord < 8,
 // compute_al, xppm.py:147-181).  q[p-2..p+1] valid.
-template <typename T>
+template <typename T, typename S>
 __device__ __forceinline__ T ppm_al(const T* q, int p, int f, int is, int ie,
-                                    const DxRow<T>& dxr) {
+                                    const DxRow<T, S>& dxr) {
   if (f == is + 1 || f == ie + 2) {
     return T(FV3_PPM_C3) * q[p - 1] + T(FV3_PPM_C2) * q[p] + T(FV3_PPM_C1) * q[p + 1];
   }
@@ -175,9 +176,9 @@ __device__ __forceinline__ T ppm_al(const T* q, int p, int f, int is, int ie,
 // Mean of q advected through interface i (compute_x_flux, xppm.py:269-280).
 // q6[n] = q of cell i-3+n; c = Courant number on the interface.
 // ORD is the reference's iord/jord (5, 6, 7 or 8).
-template <typename T, int ORD>
+template <typename T, int ORD, typename S>
 __device__ __forceinline__ T ppm_face(const T (&q6)[6], T c, int i, int is, int ie,
-                                      const DxRow<T>& dxr) {
+                                      const DxRow<T, S>& dxr) {
   if constexpr (ORD == 8) {
     T bl, br;
     if (c > T(0)) {
@@ -229,9 +230,9 @@ __device__ __forceinline__ T ppm_face(const T (&q6)[6], T c, int i, int is, int 
 // evaluation follows.  Arithmetic and rounding are those of ppm_face.  With
 // EDGES = false the tile-edge regions are not tested (blocks far from any
 // tile edge).
-template <typename T, int ORD, bool EDGES>
+template <typename T, int ORD, bool EDGES, typename S>
 __device__ __forceinline__ T ppm_face_sel(const T (&q6)[6], T c, int i, int is, int ie,
-                                          const DxRow<T>& dxr) {
+                                          const DxRow<T, S>& dxr) {
   const bool pos = c > T(0);
   if constexpr (ORD == 8) {
     T w[5];
@@ -239,7 +240,7 @@ __device__ __forceinline__ T ppm_face_sel(const T (&q6)[6], T c, int i, int is, 
     for (int m = 0; m < 5; ++m) w[m] = pos ? q6[m] : q6[m + 1];
     T bl, br;
     if constexpr (EDGES) {
-      DxRow<T> dw{dxr.base + (pos ? 0 : dxr.stride), dxr.stride, 5};
+      DxRow<T, S> dw{dxr.base + (pos ? 0 : dxr.stride), dxr.stride, 5};
       ppm_blbr8_w(w, pos ? i - 1 : i, is, ie, dw, bl, br);
     } else {
       const T dmm = ppm_dm(w[0], w[1], w[2]);

@@ -101,6 +101,39 @@ int32_t fv3_fvtp2d_f32(const fv3_domain* d, int32_t hord, const float* q_x,
                        float* q_x_flux, float* q_y_flux, void* stream);
 
 /*
+ * Tangent-linear model and adjoint of fv3_fvtp2d with respect to the
+ * transported field, at the branch decisions (limiters, smoothness masks,
+ * upwind side) of the state q_x / q_y.  No counterpart in the reference,
+ * which has no derivative code: they linearise FiniteVolumeTransport.__call__
+ * (fv3core/stencils/fvtp2d.py:249-362) for the adjoint of PPM transport.
+ * Arguments as fv3_fvtp2d; Courant numbers, area and mass fluxes and the
+ * metric terms are held fixed.
+ *   tlm:     dq_x, dq_y (perturbations of q_x, q_y) -> dq_x_flux, dq_y_flux
+ *            (written on the flux domain of fv3_fvtp2d).
+ *   adjoint: g_x_flux, g_y_flux (cotangents on the flux domain; values
+ *            outside it are ignored) -> g_q_x, g_q_y (the whole storage is
+ *            overwritten; for q_x == q_y the gradient is their sum).
+ * work: fv3_fvtp2d_workspace_slabs() slabs of work_slab elements, each a
+ * field with the strides of q_x ((ntiles - 1) * st + nk * sk elements).
+ * The compute domain needs its 3-cell halo inside the storage.  FP64 only.
+ */
+int32_t fv3_fvtp2d_workspace_slabs(void);
+int32_t fv3_fvtp2d_tlm_f64(const fv3_domain* d, int32_t hord, const double* q_x,
+                           const double* q_y, const double* crx, const double* cry,
+                           const double* x_area_flux, const double* y_area_flux,
+                           const double* x_mass_flux, const double* y_mass_flux,
+                           const double* area, const double* dxa, const double* dya,
+                           const double* dq_x, const double* dq_y, double* dq_x_flux,
+                           double* dq_y_flux, double* work, int64_t work_slab, void* stream);
+int32_t fv3_fvtp2d_adjoint_f64(const fv3_domain* d, int32_t hord, const double* q_x,
+                               const double* q_y, const double* crx, const double* cry,
+                               const double* x_area_flux, const double* y_area_flux,
+                               const double* x_mass_flux, const double* y_mass_flux,
+                               const double* area, const double* dxa, const double* dya,
+                               const double* g_x_flux, const double* g_y_flux, double* g_q_x,
+                               double* g_q_y, double* work, int64_t work_slab, void* stream);
+
+/*
  * XPiecewiseParabolic.__call__ / YPiecewiseParabolic.__call__
  * (fv3core/stencils/xppm.py:283-348, yppm.py): mean of q advected through
  * the x (direction 0) or y (direction 1) interfaces of the box
