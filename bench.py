@@ -18,6 +18,11 @@ Workload "fv_dynamics": DynamicalCore.step_dynamics on six C128 tiles
 with kord 10; cell-level updates per step = 6 * 128 * 128 * 79 (the acoustic
 count, 6x that, is given beside it).
 
+Workload "adjoint": an 8-step flux-form fv_tp_2d (hord 8) transport on one
+doubly periodic 96x96x79 tile and the gradient of J = 0.5 ||q_T - q_obs||^2
+by the hand-written adjoint; the line adds "adjoint_over_forward", the time
+of the reverse sweep (re-linearisation included) over the forward run.
+
 Workload "fvtp2d": FiniteVolumeTransport (fv_tp_2d, hord=8, area fluxes) on
 6 tiles of 96x96x79, FP64, one fused kernel per step; cell-level updates per
 step = 6 * 96 * 96 * 79.
@@ -52,7 +57,7 @@ def _parse():
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
     p.add_argument("--workload", default="dyncore",
-                   choices=["dyncore", "fv_dynamics", "fvtp2d"])
+                   choices=["dyncore", "fv_dynamics", "fvtp2d", "adjoint"])
     p.add_argument("--roofline-kernel", default="fvtp2d_kernel",
                    choices=["fvtp2d_kernel", "riem3_kernel", "riem_c_kernel",
                             "map_single_kernel"])
@@ -452,6 +457,96 @@ def bench_fvtp2d(steps: int, warmup: int, dtype_name: str, copy_calibration: boo
     return out
 
 
+def bench_adjoint(steps: int, warmup: int):
+    """Adjoint of PPM transport: one doubly periodic C96 tile 96x96x79, hord 8,
+    an 8-step flux-form transport of q (dt fixed by the Courant numbers of the
+    periodic workload) and J = 0.5 ||q_T - q_obs||^2 with q_obs the run from
+    a seeded 1e-2 perturbation of q0.  A step is one 8-step forward run; the
+    reverse sweep (per step: re-linearisation from the saved input, then the
+    adjoint kernels) is timed separately and reported as its ratio to the
+    forward run."""
+    import torch
+
+    from fv3core_amd import _lib
+    from fv3core_amd.adjoint import PeriodicTransport
+    from fv3core_amd.utils.grid import to_storage
+    from fv3core_amd.utils.stencil import GridIndexing
+    from fv3core_amd.workloads import transport_inputs
+
+    n, nz, hord, nsteps = 96, 79, 8, 8
+    d = transport_inputs(n, nz, workload=4)
+    gi = GridIndexing((n, n, nz), 3, False, False, False, False)
+    dev = {k: to_storage(v) for k, v in d.items()}
+    model = PeriodicTransport(gi, dev, hord)
+    q0 = torch.as_tensor(d["q"]).permute(2, 1, 0).contiguous().cuda()
+    g = torch.Generator(device="cpu").manual_seed(4)
+    with torch.no_grad():
+        q_obs = model.run(q0 + 1e-2 * torch.randn(q0.shape, generator=g,
+                                                  dtype=q0.dtype).cuda(), nsteps)
+
+    def forward_run():
+        with torch.no_grad():
+            return model.run(q0, nsteps)
+
+    def recorded():
+        x = q0.detach().clone().requires_grad_(True)
+        r = model.interior(model.run(x, nsteps)) - model.interior(q_obs)
+        return x, 0.5 * (r * r).sum()
+
+    for _ in range(warmup):
+        forward_run()
+        x, cost = recorded()
+        torch.autograd.grad(cost, x)
+    torch.cuda.synchronize()
+    lib = _lib.lib()
+    lib.fv3_timing_arm(b"fvtp2d_kernel")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        forward_run()
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / steps
+    cnt, kern_ms, kern_bytes = _timing(lib, "fvtp2d_kernel")
+    lib.fv3_timing_arm(None)
+    rev = []
+    for _ in range(steps):
+        x, cost = recorded()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        torch.autograd.grad(cost, x)
+        torch.cuda.synchronize()
+        rev.append(time.perf_counter() - t1)
+    rev_s = float(np.median(rev))
+    cells = n * n * nz * nsteps
+    avg_s = kern_ms / max(cnt, 1) * 1e-3
+    achieved = kern_bytes / max(kern_ms * 1e-3, 1e-30) / 1e9
+    return {
+        "value": cells / fwd,
+        "ms_per_step": fwd * 1e3,
+        "adjoint_over_forward": rev_s / fwd,
+        "reverse_ms_per_step": rev_s * 1e3,
+        "config": {
+            "workload": f"fv_tp_2d hord={hord} adjoint: one doubly periodic tile {n}x{n}x{nz}, "
+            f"{nsteps}-step flux-form transport, J = 0.5||q_T - q_obs||^2, reverse sweep "
+            "re-linearising each step from its saved input (map_scalar remap adjoint not built)",
+            "tiles": 1, "n": n, "nz": nz, "hord": hord, "transport_steps": nsteps,
+            "cells_per_step": cells,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": "fvtp2d_kernel",
+            "kernel_launches_timed": cnt,
+            "kernel_avg_us": avg_s * 1e6,
+            "algorithmic_bytes_per_launch": kern_bytes / max(cnt, 1),
+        },
+        "dtype": "f64",
+    }
+
+
 def main():
     args = _parse()
     if args.gpus != 1:
@@ -469,6 +564,11 @@ def main():
         steps = 5 if args.steps is None else args.steps
         warmup = 1 if args.warmup is None else args.warmup
         res = bench_fv_dynamics(steps, warmup, args.roofline_kernel)
+        cpu = None
+    elif args.workload == "adjoint":
+        steps = 10 if args.steps is None else args.steps
+        warmup = 2 if args.warmup is None else args.warmup
+        res = bench_adjoint(steps, warmup)
         cpu = None
     else:
         steps = 200 if args.steps is None else args.steps
@@ -493,7 +593,8 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
     }
-    for extra in ("eager_ms_per_step", "graph_ms_per_step"):
+    for extra in ("eager_ms_per_step", "graph_ms_per_step", "adjoint_over_forward",
+                  "reverse_ms_per_step"):
         if res.get(extra) is not None:
             line[extra] = res[extra]
     if res.get("copy_calibration_bytes"):

@@ -23,7 +23,7 @@ autograd-side state is kept as row-major ``(K, J, I)`` tensors, whose
 from __future__ import annotations
 
 import ctypes
-from typing import Optional, Tuple
+from typing import Tuple
 
 import torch
 import torch.nn.functional as F
@@ -44,10 +44,11 @@ def as_ijk_storage(t: torch.Tensor) -> torch.Tensor:
     return ijk_view(t.permute(2, 1, 0).contiguous())
 
 
-def new_ijk(like: torch.Tensor) -> torch.Tensor:
-    """Zeroed (I, J, K) i-fastest storage shaped like ``like``."""
+def new_ijk(like: torch.Tensor, zero: bool = True) -> torch.Tensor:
+    """(I, J, K) i-fastest storage shaped like ``like`` (zeroed unless told not to)."""
     ni, nj, nk = like.shape
-    return ijk_view(torch.zeros((nk, nj, ni), dtype=like.dtype, device=like.device))
+    make = torch.zeros if zero else torch.empty
+    return ijk_view(make((nk, nj, ni), dtype=like.dtype, device=like.device))
 
 
 class TransportLinearization:
@@ -114,7 +115,7 @@ class TransportLinearization:
     def adjoint(self, qx, qy, crx, cry, xfx, yfx, gfx, gfy, xmf=None, ymf=None):
         """(g_qx, g_qy): the transposed linearisation applied to (gfx, gfy)."""
         _lib.require_device(qx, qy, crx, cry, xfx, yfx, xmf, ymf, gfx, gfy)
-        gqx, gqy = new_ijk(qx), new_ijk(qx)
+        gqx, gqy = new_ijk(qx, zero=False), new_ijk(qx, zero=False)  # fully overwritten
         d = self._domain(qx, qy, crx, cry, xfx, yfx, xmf, ymf, gfx, gfy, gqx, gqy)
         work, slab = self._workspace(qx)
         _lib.check(_lib.lib().fv3_fvtp2d_adjoint_f64(
@@ -178,16 +179,23 @@ class PeriodicTransport:
                                           hord, nk)
         self.aux = (crx, as_ijk_storage(inputs["cry"]), as_ijk_storage(inputs["x_area_flux"]),
                     as_ijk_storage(inputs["y_area_flux"]), None, None)
-        dev = crx.device
         h, n = self.h, self.n
-        self._ii = (h + (torch.arange(ni, device=dev) - h) % n)
-        self._jj = (h + (torch.arange(nj, device=dev) - h) % n)
+        if n < max(h, ni - h - n, nj - h - n):
+            raise ValueError("the tile is narrower than its halo")
         rarea = 1.0 / inputs["area"]                       # (I, J)
         self._rarea = rarea.t()[None, h:h + n, h:h + n]   # (1, J, I) interior
 
     def fill(self, q_kji: torch.Tensor) -> torch.Tensor:
-        """Halo of a (K, J, I) state refilled by wrapping (differentiable)."""
-        return q_kji[:, self._jj][:, :, self._ii]
+        """Halo of a (K, J, I) state refilled by wrapping (differentiable).
+
+        Built from slices and concatenations, whose backward is slicing and
+        addition (no index sort as for a general gather).
+        """
+        h, n = self.h, self.n
+        ni, nj, _ = self.shape
+        c = q_kji[:, h:h + n, h:h + n]
+        c = torch.cat([c[:, n - h:n], c, c[:, 0:nj - h - n]], dim=1)
+        return torch.cat([c[:, :, n - h:n], c, c[:, :, 0:ni - h - n]], dim=2)
 
     def step(self, q_kji: torch.Tensor) -> torch.Tensor:
         h, n = self.h, self.n

@@ -13,6 +13,11 @@
 
 namespace fv3 {
 
+// keep the double overloads visible next to the Dual ones below
+using ::fabs;
+using ::fmax;
+using ::fmin;
+
 template <int N>
 struct Dual {
   double v;

@@ -16,11 +16,11 @@
 // Each 1-D PPM face depends on the six cells around it.  Its six partial
 // derivatives are obtained by running the forward's own face function
 // (ppm_face, every hord and tile-edge branch) on Dual<6> numbers, whose value
-// part repeats the double arithmetic exactly.  The linearisation pass stores
-// the partials and the intermediate fields in workspace slabs; the TLM then
-// applies the chain forward, the adjoint applies its transpose as gathers
-// (each output point sums its own contributions, no atomics), so both are
-// deterministic.
+// part repeats the double arithmetic exactly.  The TLM stores the partials and
+// the intermediate fields in workspace slabs and applies the chain forward
+// (deterministic).  The adjoint keeps no partials: each face evaluates them
+// and adds its contributions into the cotangent fields (see
+// launch_adjoint_fused).
 #include "common.hpp"
 #include "dual.hpp"
 #include "ppm.hpp"
@@ -208,100 +208,127 @@ void launch_tlm(hipStream_t s, const Lin& L, const double* dqx, const double* dq
              });
 }
 
-// Adjoint: g_qx, g_qy from g_fx, g_fy (overwrites the whole storage of both).
-void launch_adjoint(hipStream_t s, const Lin& L, const double* gfx, const double* gfy,
-                    double* gqx, double* gqy) {
+// Adjoint without stored partials.  (1) the forward values q_i and q_j are
+// recomputed (plain double faces) and the accumulators zeroed; (2) every
+// outer face evaluates its six partials on Dual<6> and adds its cotangent
+// times each partial into the cotangents of q_i / q_j; (3) every inner face
+// forms its cotangent (its own flux share plus the transposed cross update,
+// gathered from the completed q_i / q_j cotangents), evaluates its partials
+// and adds into g_qy / g_qx, and each cell adds the direct cross-update
+// term.  The adds are hardware FP64 atomics (at most 13 per cell), so the
+// summation order, and the last bit of the result, may differ between runs.
+template <int ORD>
+void launch_adjoint_fused(hipStream_t s, const Lin& L, const double* gfx, const double* gfy,
+                          double* gqx, double* gqy) {
   const Dom d = L.d;
   const int I0 = d.isc, I1 = d.iec, J0 = d.jsc, J1 = d.jec;
   const Box wide{I0 - 3, I1 + 3, J0 - 3, J1 + 3};
-  // cotangents of the outer-sweep inputs q_i and q_j
+  launch_box(s, d, Box{0, d.ni - 1, 0, d.nj - 1}, 0, d.nk,
+             [=] __device__(int i, int j, int k, int t) {
+               const int64_t o = at3(d, i, j, k, t);
+               gqx[o] = 0.0;
+               gqy[o] = 0.0;
+               L.s(kLi)[o] = 0.0;
+               L.s(kLj)[o] = 0.0;
+               if (i < I0 - 3 || i > I1 + 3 || j < J0 - 3 || j > J1 + 3) return;
+               const int64_t o2 = at2(d, i, j, t);
+               const double a = L.area[o2];
+               if (j >= J0 && j <= J1) {  // q_i from the y-inner faces j, j+1
+                 double f[2];
+#pragma unroll
+                 for (int u = 0; u < 2; ++u) {
+                   const int64_t of = o + u * d.sj;
+                   double q6[6];
+#pragma unroll
+                   for (int m = 0; m < 6; ++m) q6[m] = L.qy[of + (m - 3) * d.sj];
+                   DxRow<double> dr{L.dya + o2 + (int64_t)(u - 3) * d.sj2, d.sj2};
+                   f[u] = ppm_face<double, ORD>(q6, L.cry[of], j + u, d.js, d.je, dr);
+                 }
+                 const double g0 = L.ay[o], g1 = L.ay[o + d.sj];
+                 L.s(kQi)[o] = (L.qy[o] * a + g0 * f[0] - g1 * f[1]) / (a + g0 - g1);
+               }
+               if (i >= I0 && i <= I1) {  // q_j from the x-inner faces i, i+1
+                 double f[2];
+#pragma unroll
+                 for (int u = 0; u < 2; ++u) {
+                   const int64_t of = o + u;
+                   double q6[6];
+#pragma unroll
+                   for (int m = 0; m < 6; ++m) q6[m] = L.qx[of + (m - 3)];
+                   DxRow<double> dr{L.dxa + o2 + (u - 3), 1};
+                   f[u] = ppm_face<double, ORD>(q6, L.crx[of], i + u, d.is, d.ie, dr);
+                 }
+                 const double f0 = L.ax[o], f1 = L.ax[o + 1];
+                 L.s(kQj)[o] = (L.qx[o] * a + f0 * f[0] - f1 * f[1]) / (a + f0 - f1);
+               }
+             });
+  // outer faces -> cotangents of q_i, q_j
+  launch_box(s, d, Box{I0, I1 + 1, J0, J1 + 1}, 0, d.nk,
+             [=] __device__(int i, int j, int k, int t) {
+               const int64_t o = at3(d, i, j, k, t);
+               const int64_t o2 = at2(d, i, j, t);
+               if (j <= J1) {
+                 const double g = 0.5 * L.ux[o] * gfx[o];
+                 const D6 v = face_dual<ORD>(L.s(kQi) + o, 1, L.crx[o], i, d.is, d.ie,
+                                             L.dxa + o2, 1);
+#pragma unroll
+                 for (int m = 0; m < 6; ++m) unsafeAtomicAdd(L.s(kLi) + o + (m - 3), g * v.d[m]);
+               }
+               if (i <= I1) {
+                 const double g = 0.5 * L.uy[o] * gfy[o];
+                 const D6 v = face_dual<ORD>(L.s(kQj) + o, d.sj, L.cry[o], j, d.js, d.je,
+                                             L.dya + o2, d.sj2);
+#pragma unroll
+                 for (int m = 0; m < 6; ++m)
+                   unsafeAtomicAdd(L.s(kLj) + o + (int64_t)(m - 3) * d.sj, g * v.d[m]);
+               }
+             });
+  // inner faces and the cross updates -> g_qy, g_qx
   launch_box(s, d, wide, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
     const int64_t o = at3(d, i, j, k, t);
-    if (j >= J0 && j <= J1) {  // q_i(i, j) feeds x-outer faces f = i + 3 - m
-      double g = 0.0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        const int f = i + 3 - m;
-        if (f >= I0 && f <= I1 + 1) {
-          const int64_t of = o + (3 - m);
-          g += L.s(kAxo + m)[of] * (0.5 * L.ux[of] * gfx[of]);
-        }
-      }
-      L.s(kLi)[o] = g;
+    const int64_t o2 = at2(d, i, j, t);
+    const double a = L.area[o2];
+    if (j >= J0 && j <= J1) {  // direct term of q_i(i, j) on q_y(i, j)
+      unsafeAtomicAdd(gqy + o, L.s(kLi)[o] * a / (a + L.ay[o] - L.ay[o + d.sj]));
     }
-    if (i >= I0 && i <= I1) {  // q_j(i, j) feeds y-outer faces f = j + 3 - m
-      double g = 0.0;
-#pragma unroll
-      for (int m = 0; m < 6; ++m) {
-        const int f = j + 3 - m;
-        if (f >= J0 && f <= J1 + 1) {
-          const int64_t of = o + (int64_t)(3 - m) * d.sj;
-          g += L.s(kAyo + m)[of] * (0.5 * L.uy[of] * gfy[of]);
-        }
-      }
-      L.s(kLj)[o] = g;
+    if (i >= I0 && i <= I1) {  // direct term of q_j(i, j) on q_x(i, j)
+      unsafeAtomicAdd(gqx + o, L.s(kLj)[o] * a / (a + L.ax[o] - L.ax[o + 1]));
     }
-  });
-  // cotangents of the inner faces
-  launch_box(s, d, wide, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
     if (j >= J0 && j <= J1 + 1) {  // y-inner face (i, j)
       double g = (i >= I0 && i <= I1) ? 0.5 * L.uy[o] * gfy[o] : 0.0;
       const double ayf = L.ay[o];
-      if (j <= J1) {
-        const double a = L.area[at2(d, i, j, t)];
-        g += ayf * L.s(kLi)[o] / (a + ayf - L.ay[o + d.sj]);
-      }
+      if (j <= J1) g += ayf * L.s(kLi)[o] / (a + ayf - L.ay[o + d.sj]);
       if (j - 1 >= J0) {
-        const double a = L.area[at2(d, i, j - 1, t)];
-        g -= ayf * L.s(kLi)[o - d.sj] / (a + L.ay[o - d.sj] - ayf);
+        const double am = L.area[o2 - d.sj2];
+        g -= ayf * L.s(kLi)[o - d.sj] / (am + L.ay[o - d.sj] - ayf);
       }
-      L.s(kLy2)[o] = g;
+      const D6 v = face_dual<ORD>(L.qy + o, d.sj, L.cry[o], j, d.js, d.je, L.dya + o2, d.sj2);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) unsafeAtomicAdd(gqy + o + (int64_t)(m - 3) * d.sj, g * v.d[m]);
     }
     if (i >= I0 && i <= I1 + 1) {  // x-inner face (i, j)
       double g = (j >= J0 && j <= J1) ? 0.5 * L.ux[o] * gfx[o] : 0.0;
       const double axf = L.ax[o];
-      if (i <= I1) {
-        const double a = L.area[at2(d, i, j, t)];
-        g += axf * L.s(kLj)[o] / (a + axf - L.ax[o + 1]);
-      }
+      if (i <= I1) g += axf * L.s(kLj)[o] / (a + axf - L.ax[o + 1]);
       if (i - 1 >= I0) {
-        const double a = L.area[at2(d, i - 1, j, t)];
-        g -= axf * L.s(kLj)[o - 1] / (a + L.ax[o - 1] - axf);
+        const double am = L.area[o2 - 1];
+        g -= axf * L.s(kLj)[o - 1] / (am + L.ax[o - 1] - axf);
       }
-      L.s(kLx2)[o] = g;
+      const D6 v = face_dual<ORD>(L.qx + o, 1, L.crx[o], i, d.is, d.ie, L.dxa + o2, 1);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) unsafeAtomicAdd(gqx + o + (m - 3), g * v.d[m]);
     }
   });
-  // cotangents of q_y and q_x over the whole storage
-  launch_box(s, d, Box{0, d.ni - 1, 0, d.nj - 1}, 0, d.nk,
-             [=] __device__(int i, int j, int k, int t) {
-               const int64_t o = at3(d, i, j, k, t);
-               const bool iw = i >= I0 - 3 && i <= I1 + 3, jw = j >= J0 - 3 && j <= J1 + 3;
-               double gy = 0.0, gx = 0.0;
-               if (iw && jw) {
-                 const double a = L.area[at2(d, i, j, t)];
-                 if (j >= J0 && j <= J1) gy = L.s(kLi)[o] * a / (a + L.ay[o] - L.ay[o + d.sj]);
-#pragma unroll
-                 for (int m = 0; m < 6; ++m) {  // y-inner faces f = j + 3 - m
-                   const int f = j + 3 - m;
-                   if (f >= J0 && f <= J1 + 1) {
-                     const int64_t of = o + (int64_t)(3 - m) * d.sj;
-                     gy += L.s(kAy + m)[of] * L.s(kLy2)[of];
-                   }
-                 }
-                 if (i >= I0 && i <= I1) gx = L.s(kLj)[o] * a / (a + L.ax[o] - L.ax[o + 1]);
-#pragma unroll
-                 for (int m = 0; m < 6; ++m) {  // x-inner faces f = i + 3 - m
-                   const int f = i + 3 - m;
-                   if (f >= I0 && f <= I1 + 1) {
-                     const int64_t of = o + (3 - m);
-                     gx += L.s(kAx + m)[of] * L.s(kLx2)[of];
-                   }
-                 }
-               }
-               gqy[o] = gy;
-               gqx[o] = gx;
-             });
+}
+
+void adjoint_fused(hipStream_t s, const Lin& L, const double* gfx, const double* gfy,
+                   double* gqx, double* gqy) {
+  switch (L.hord) {
+    case 5: launch_adjoint_fused<5>(s, L, gfx, gfy, gqx, gqy); break;
+    case 6: launch_adjoint_fused<6>(s, L, gfx, gfy, gqx, gqy); break;
+    case 7: launch_adjoint_fused<7>(s, L, gfx, gfy, gqx, gqy); break;
+    default: launch_adjoint_fused<8>(s, L, gfx, gfy, gqx, gqy); break;
+  }
 }
 
 int setup(const fv3_domain* dom, const char* who, int hord, const double* q_x, const double* q_y,
@@ -382,8 +409,7 @@ int32_t fv3_fvtp2d_adjoint_f64(const fv3_domain* d, int32_t hord, const double* 
   FV3_REQUIRE(g_x_flux && g_y_flux && g_q_x && g_q_y, FV3_E_ARG, "%s: NULL cotangent", who);
   if (d->nk == 0 || d->ntiles == 0) return FV3_OK;
   hipStream_t s = as_stream(stream);
-  linearize(s, L);
-  launch_adjoint(s, L, g_x_flux, g_y_flux, g_q_x, g_q_y);
+  adjoint_fused(s, L, g_x_flux, g_y_flux, g_q_x, g_q_y);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
 }
