@@ -18,6 +18,10 @@ Workload "fv_dynamics": DynamicalCore.step_dynamics on six C128 tiles
 with kord 10; cell-level updates per step = 6 * 128 * 128 * 79 (the acoustic
 count, 6x that, is given beside it).
 
+Workload "ensemble": 32 perturbed members of six C48 tiles in one batch of
+192 tiles plus four uneven members (C24, C64, C96, C128), AcousticDynamics
+on every batch per step, FP64 (--graph replays each batch's HIP graph).
+
 Workload "adjoint": an 8-step flux-form fv_tp_2d (hord 8) transport on one
 doubly periodic 96x96x79 tile and the gradient of J = 0.5 ||q_T - q_obs||^2
 by the hand-written adjoint; the line adds "adjoint_over_forward", the time
@@ -57,7 +61,7 @@ def _parse():
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
     p.add_argument("--workload", default="dyncore",
-                   choices=["dyncore", "fv_dynamics", "fvtp2d", "adjoint"])
+                   choices=["dyncore", "fv_dynamics", "fvtp2d", "adjoint", "ensemble"])
     p.add_argument("--roofline-kernel", default="fvtp2d_kernel",
                    choices=["fvtp2d_kernel", "riem3_kernel", "riem_c_kernel",
                             "map_single_kernel"])
@@ -457,6 +461,88 @@ def bench_fvtp2d(steps: int, warmup: int, dtype_name: str, copy_calibration: boo
     return out
 
 
+def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str):
+    """Batched ensemble: 32 members of six C48 tiles x 79 levels (the C96
+    workload's initial state at C48, each member with seeded N(0, 0.5 K) pt
+    and N(0, 1 m/s) u, v perturbations) in one batch of 192 tiles, plus four
+    uneven members at C24, C64, C96 and C128, each its own batch; FP64.  A
+    step is one AcousticDynamics call (n_split = 6, dt = 225 s) of every
+    batch; cell-level updates per step = sum over members of 6 n^2 79 6."""
+    import torch
+
+    from fv3core_amd import _lib
+    from fv3core_amd._config import DynamicalCoreConfig
+    from fv3core_amd.ensemble import UnevenEnsemble, ensemble_acoustic
+    from fv3core_amd.workloads import cube_state
+
+    nz, n_split = 79, 6
+    cfg = DynamicalCoreConfig(npz=nz, n_split=n_split, k_split=1, nord=3, dt_atmos=225)
+    plan = [(48, 32)] + [(n, 1) for n in (24, 64, 96, 128)]
+    parts, cells = [], 0
+    for n, members in plan:
+        init = cube_state(n, nz, seed=0)
+        parts.append(ensemble_acoustic(init, members, cfg, seed=1000 * n))
+        cells += members * 6 * n * n * nz * n_split
+        del init
+    ens = UnevenEnsemble(parts)
+    lib = _lib.lib()
+    for _ in range(warmup):
+        ens()
+    torch.cuda.synchronize()
+    lib.fv3_timing_arm(roofline_kernel.encode())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cnt, kern_ms, kern_bytes = _timing(lib, roofline_kernel)
+    lib.fv3_timing_arm(None)
+    graph_wall = None
+    if graph:  # the same batches replayed as HIP graphs (eager, capture, then replays)
+        for acoustic, _ in parts:
+            acoustic._use_graph = True
+        for _ in range(3):
+            ens()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ens()
+        torch.cuda.synchronize()
+        graph_wall = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(st.pt[:, 3:-4, 3:-4, :nz]).all().item()) for _, st in parts)
+    avg_s = kern_ms / max(cnt, 1) * 1e-3
+    achieved = kern_bytes / max(kern_ms * 1e-3, 1e-30) / 1e9
+    used = graph_wall if graph_wall else wall
+    return {
+        "value": steps * cells / used,
+        "ms_per_step": used / steps * 1e3,
+        "eager_ms_per_step": wall / steps * 1e3,
+        "graph_ms_per_step": graph_wall / steps * 1e3 if graph_wall else None,
+        "config": {
+            "workload": "batched ensemble: 32 members x 6 x C48 x 79 in one batch + uneven "
+            "members C24/C64/C96/C128 (one cube each), AcousticDynamics n_split=6, dt=225 s, "
+            "perturbed jet states, FP64" + (", each batch replayed as a HIP graph" if graph
+                                            else ""),
+            "members": 36, "tiles": 6 * 36, "nz": nz, "n_split": n_split,
+            "cells_per_step": cells, "state_finite_after_timed_steps": finite,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": None,
+            "kernel": roofline_kernel,
+            "kernel_launches_timed": cnt,
+            "kernel_avg_us": avg_s * 1e6,
+            "algorithmic_bytes_per_launch": kern_bytes / max(cnt, 1),
+        },
+        "dtype": "f64",
+    }
+
+
 def bench_adjoint(steps: int, warmup: int):
     """Adjoint of PPM transport: one doubly periodic C96 tile 96x96x79, hord 8,
     an 8-step flux-form transport of q (dt fixed by the Courant numbers of the
@@ -564,6 +650,11 @@ def main():
         steps = 5 if args.steps is None else args.steps
         warmup = 1 if args.warmup is None else args.warmup
         res = bench_fv_dynamics(steps, warmup, args.roofline_kernel)
+        cpu = None
+    elif args.workload == "ensemble":
+        steps = 3 if args.steps is None else args.steps
+        warmup = 1 if args.warmup is None else args.warmup
+        res = bench_ensemble(steps, warmup, args.graph, args.roofline_kernel)
         cpu = None
     elif args.workload == "adjoint":
         steps = 10 if args.steps is None else args.steps

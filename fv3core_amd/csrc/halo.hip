@@ -6,7 +6,9 @@
 // index map built on the host (fv3core_amd/halo.py): for each halo point,
 // the source tile, source indices, source component and sign.  One thread
 // handles one (map entry, level) and moves that value for every field group
-// of the call.
+// of the call.  A batch of several cubes (ensemble members, six consecutive
+// tiles each) applies the same six-tile map to every member: grid.y is the
+// member and offsets both tile indices by six per member.
 #include "common.hpp"
 
 namespace fv3 {
@@ -24,8 +26,9 @@ __global__ void __launch_bounds__(256)
   const int e = (int)(n % nent);
   const int k = (int)(n / nent);
   const int* r = mp + 8 * (int64_t)e;
-  const int t = r[0], i = r[1], j = r[2], c = r[3];
-  const int tb = r[4], ib = r[5], jb = r[6], sc = r[7];
+  const int m6 = 6 * (int)blockIdx.y;  // first tile of this member
+  const int t = m6 + r[0], i = r[1], j = r[2], c = r[3];
+  const int tb = m6 + r[4], ib = r[5], jb = r[6], sc = r[7];
   const int src_comp = sc >> 1;
   const bool neg = (sc & 1) != 0;
   const int64_t od = (int64_t)t * d.st + (int64_t)k * d.sk + (int64_t)j * d.sj + i;
@@ -48,15 +51,15 @@ static int halo_launch(const fv3_domain* dom, int nent, const int* mp, int nfiel
   FV3_REQUIRE(ncomp == 1 || ncomp == 2, FV3_E_ARG, "%s: ncomp must be 1 or 2", who);
   FV3_REQUIRE(nfields >= 1 && nfields <= 16 && nfields % ncomp == 0, FV3_E_ARG,
               "%s: bad field count %d", who, nfields);
-  FV3_REQUIRE(dom->ntiles == 6, FV3_E_ARG, "%s: the cube has 6 tiles (got %d)", who,
-              dom->ntiles);
+  FV3_REQUIRE(dom->ntiles >= 6 && dom->ntiles % 6 == 0 && dom->ntiles / 6 <= 65535, FV3_E_ARG,
+              "%s: the batch must hold whole six-tile cubes (got %d tiles)", who, dom->ntiles);
   for (int a = 0; a < nfields; ++a)
     FV3_REQUIRE(fp->p[a] != nullptr, FV3_E_ARG, "%s: field %d is NULL", who, a);
   if (nent == 0 || dom->nk == 0) return FV3_OK;
   Dom d = to_dom(dom);
   const int64_t total = (int64_t)nent * d.nk;
-  hipLaunchKernelGGL((halo_kernel<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                     as_stream(stream), d, nent, mp, nfields / ncomp, ncomp, *fp);
+  hipLaunchKernelGGL((halo_kernel<T>), dim3((unsigned)((total + 255) / 256), d.ntiles / 6),
+                     dim3(256), 0, as_stream(stream), d, nent, mp, nfields / ncomp, ncomp, *fp);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
 }

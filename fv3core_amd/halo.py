@@ -255,8 +255,8 @@ class HaloUpdater:
             raise ValueError(f"at most {_MAX_FIELDS} fields per update")
         _lib.require_device(*fields)
         ref = fields[0]
-        if ref.dim() != 4 or ref.shape[0] != 6:
-            raise ValueError("halo updates act on batched (6, I, J, K) storages")
+        if ref.dim() != 4 or ref.shape[0] % 6 != 0 or ref.shape[0] == 0:
+            raise ValueError("halo updates act on batched (6 * members, I, J, K) storages")
         nk = min(f.shape[-1] for f in fields)
         d = _lib.make_domain(self._gi, ref, nk, others3=fields[1:])
         mp = self._device_map(ref.device)

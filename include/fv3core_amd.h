@@ -223,7 +223,9 @@ int32_t fv3_riem_solver3_f64(const fv3_domain* d, int32_t last_call, double dt, 
  * nentries x 8 int32 (dst_tile, dst_i, dst_j, dst_comp, src_tile, src_i,
  * src_j, src_comp*2 + negate), built by fv3core_amd/halo.py.  fields: host
  * pointer to 16 device pointers; nfields of them are used, in groups of
- * ncomp (1 scalar, 2 = x/y vector components).  d->ntiles must be 6.
+ * ncomp (1 scalar, 2 = x/y vector components).  d->ntiles is a multiple
+ * of 6: a batch of cubes (ensemble members, six consecutive tiles each),
+ * each updated with the same six-tile map (tile indices 0..5).
  */
 int32_t fv3_halo_update_f64(const fv3_domain* d, int32_t nentries, const int32_t* map,
                             int32_t nfields, int32_t ncomp, const void* fields, void* stream);
