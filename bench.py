@@ -68,6 +68,9 @@ def _parse():
     p.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     p.add_argument("--graph", action="store_true",
                    help="dyncore: also time the step replayed from a captured HIP graph")
+    p.add_argument("--mixed", action="store_true",
+                   help="dyncore: FP32 arithmetic in d_sw's transports; reports the error of "
+                        "every field against the FP64 form")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--copy-calibration", action="store_true",
@@ -197,8 +200,34 @@ def _timing(lib, name):
     return cnt.value, ms.value, byt.value
 
 
+def _mixed_errors(make, state_of, n, nz):
+    """Max 2|a-b|/(|a|+|b|) (denominators floored at 1e-6 of the field's max)
+    of every output field after one AcousticDynamics call of the mixed form
+    against the FP64 form from the same initial state; ps is pe at the
+    surface."""
+    import torch
+
+    plain, mixed = make(False), make(True)
+    sp, sm = state_of(), state_of()
+    plain(sp)
+    mixed(sm)
+    torch.cuda.synchronize()
+    c = (slice(None), slice(3, n + 3), slice(3, n + 3))
+
+    def rel(a, b):
+        scale = float(torch.maximum(a.abs().max(), b.abs().max()))
+        den = torch.clamp(a.abs() + b.abs(), min=1e-6 * scale + 1e-300)
+        return float((2.0 * (a - b).abs() / den).max())
+
+    errs = {k: rel(getattr(sm, k)[c + (slice(0, nz),)], getattr(sp, k)[c + (slice(0, nz),)])
+            for k in ("delp", "pt", "delz", "w", "u", "v", "zh", "pe", "pk", "peln", "pkz", "ua",
+                      "va", "uc", "vc", "omga")}
+    errs["ps"] = rel(sm.pe[c + (nz,)], sp.pe[c + (nz,)])
+    return errs
+
+
 def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = False,
-                  copy_calibration: bool = False):
+                  copy_calibration: bool = False, mixed: bool = False):
     import ctypes
 
     import torch
@@ -231,12 +260,21 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
     damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
     comm = CubedSphereCommunicator(gi)
     phis = dev(init["phis"], 1)
-    acoustic = AcousticDynamics(comm, factory, grid, damp, 0, False, False,
-                                cfg.acoustic_dynamics, ak, bk, init_pfull(ak, bk, 1.0e5), phis)
-    state = types.SimpleNamespace(**{k: dev(v, None) for k, v in st0.items()})
-    state.phis = phis
-    state.wsd = dev(init["wsd"], 1)
-    state.ptop, state.ks, state.mdt, state.n_map = ptop, 0, float(cfg.dt_atmos), 1
+    def make(mixed_form):
+        return AcousticDynamics(comm, factory, grid, damp, 0, False, False,
+                                cfg.acoustic_dynamics, ak, bk, init_pfull(ak, bk, 1.0e5), phis,
+                                mixed_precision=mixed_form)
+
+    def state_of():
+        st = types.SimpleNamespace(**{k: dev(v, None) for k, v in st0.items()})
+        st.phis = phis
+        st.wsd = dev(init["wsd"], 1)
+        st.ptop, st.ks, st.mdt, st.n_map = ptop, 0, float(cfg.dt_atmos), 1
+        return st
+
+    mixed_errs = _mixed_errors(make, state_of, n, nz) if mixed else None
+    acoustic = make(mixed)
+    state = state_of()
     lib = _lib.lib()
     for _ in range(warmup):
         acoustic(state)
@@ -273,7 +311,9 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
         "graph_ms_per_step": graph_wall / steps * 1e3 if graph_wall else None,
         "config": {
             "workload": f"AcousticDynamics (dyn_core) on 6 cube tiles x {n}x{n}x{nz}, "
-            f"n_split={n_split}, dt=225 s, nonhydrostatic, jet + 1 K pt bump initial state",
+            f"n_split={n_split}, dt=225 s, nonhydrostatic, jet + 1 K pt bump initial state" +
+            (", mixed precision: FP32 arithmetic in d_sw's fv_tp_2d transports, FP64 storages "
+             "and every other kernel FP64" if mixed else ""),
             "tiles": 6, "n": n, "nz": nz, "n_split": n_split,
             "cells_per_step": cells, "state_finite_after_timed_steps": finite,
         },
@@ -289,8 +329,11 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
             "kernel_avg_us": avg_s * 1e6,
             "algorithmic_bytes_per_launch": kern_bytes / max(cnt, 1),
         },
-        "dtype": "f64",
+        "dtype": "f64+f32" if mixed else "f64",
         "copy_calibration_bytes": copy_bytes,
+        "mixed_vs_fp64": mixed_errs,
+        "mixed_gate_ok": (max(mixed_errs[k] for k in ("ps", "pt", "delp")) <= 1e-5
+                          if mixed else None),
     }
 
 
@@ -644,7 +687,7 @@ def main():
         steps = 20 if args.steps is None else args.steps
         warmup = 3 if args.warmup is None else args.warmup
         res = bench_dyncore(steps, warmup, args.roofline_kernel, args.graph,
-                            args.copy_calibration)
+                            args.copy_calibration, args.mixed)
         cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
     elif args.workload == "fv_dynamics":
         steps = 5 if args.steps is None else args.steps
@@ -685,7 +728,7 @@ def main():
         "cpu_baseline": cpu,
     }
     for extra in ("eager_ms_per_step", "graph_ms_per_step", "adjoint_over_forward",
-                  "reverse_ms_per_step"):
+                  "reverse_ms_per_step", "mixed_vs_fp64", "mixed_gate_ok"):
         if res.get(extra) is not None:
             line[extra] = res[extra]
     if res.get("copy_calibration_bytes"):

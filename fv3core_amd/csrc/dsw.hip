@@ -865,8 +865,9 @@ int32_t fv3_d_sw_f64(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_
                        double* oy) -> int {
     int r = fv3_copy_corners_xy_f64(&d1, q, q, qy, stream);
     if (r) return r;
-    r = fv3_fvtp2d_f64(&d1, hord, q, qy, st->crx, st->cry, st->xfx, st->yfx, mfx_, mfy_, area,
-                       dxa, dya, ox, oy, stream);
+    r = (p->transport_fp32 ? fv3_fvtp2d_mixed_f64 : fv3_fvtp2d_f64)(
+        &d1, hord, q, qy, st->crx, st->cry, st->xfx, st->yfx, mfx_, mfy_, area, dxa, dya, ox, oy,
+        stream);
     if (r) return r;
     if (nord4) {
       r = fv3_delnflux_f64(dom, mass ? FV3_DELN_MASS : FV3_DELN_ADD, nord4, q, mass, damp,

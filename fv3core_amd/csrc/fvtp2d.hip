@@ -147,9 +147,9 @@ __device__ __forceinline__ T face8(T c, T qm, T blm, T brm, T q0, T bl0, T br0) 
 // PPM along i on one row held across the wave: lane L has cell i (= face i
 // on its left) with q, the Courant number c of face i and (tile-edge waves)
 // the width dxv of cell i; dxrow points at dx of column 0 of the row.
-template <typename T, int ORD, bool EDGES>
+template <typename T, int ORD, bool EDGES, typename S>
 __device__ __forceinline__ T ppm_row(T q, T c, int i, int is, int ie, T dxv,
-                                     const T* __restrict__ dxrow) {
+                                     const S* __restrict__ dxrow) {
   const T qm1 = lane_up(q, 1), qm2 = lane_up(q, 2), qp1 = lane_dn(q, 1);
   if constexpr (ORD < 8) {
     T dm2 = T(0), dm1 = T(0), dp1 = T(0);
@@ -171,7 +171,7 @@ __device__ __forceinline__ T ppm_row(T q, T c, int i, int is, int ie, T dxv,
     T bl, br;
     if constexpr (EDGES) {
       const T w[5] = {qm2, qm1, q, qp1, qp2};
-      DxRow<T> dw{dxrow + (i - 2), 1, 5};
+      DxRow<T, S> dw{dxrow + (i - 2), 1, 5};
       ppm_blbr8_w(w, i, is, ie, dw, bl, br);
     } else {
       const T dm = ppm_dm(qm1, q, qp1);
@@ -192,8 +192,9 @@ struct ColumnPPM {
   T bl0 = 0, br0 = 0, bl1 = 0, br1 = 0;  // cells c-3, c-2
   bool sm0 = false, sm1 = false;
 
+  template <typename S>
   __device__ __forceinline__ void push(T qc, T dc, int c, int is, int ie,
-                                       const T* __restrict__ dycol, int64_t sj2) {
+                                       const S* __restrict__ dycol, int64_t sj2) {
 #pragma unroll
     for (int n = 0; n < 4; ++n) q[n] = q[n + 1];
     q[4] = qc;
@@ -215,7 +216,7 @@ struct ColumnPPM {
       dm2 = ppm_dm(q[2], q[3], q[4]);
       if constexpr (EDGES) {
         const T w[5] = {q[0], q[1], q[2], q[3], q[4]};
-        DxRow<T> dwr{dycol + (int64_t)(c - 4) * sj2, sj2, 5};
+        DxRow<T, S> dwr{dycol + (int64_t)(c - 4) * sj2, sj2, 5};
         ppm_blbr8_w(w, c - 2, is, ie, dwr, bl1, br1);
       } else {
         cell8(q[1], q[2], q[3], dm0, dm1, dm2, bl1, br1);
@@ -232,13 +233,15 @@ struct ColumnPPM {
   }
 };
 
-template <typename T, int ORD, bool EX, bool EY>
+// S is the storage type of every field, T the type the march computes in
+// (the same, or float for FP64 storage in the mixed-precision form).
+template <typename S, typename T, int ORD, bool EX, bool EY>
 __device__ __forceinline__ void fvtp2d_strip(
-    const Dom& d, int i0, int j0, const T* __restrict__ qx, const T* __restrict__ qy,
-    const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ ax,
-    const T* __restrict__ ay, const T* __restrict__ ux, const T* __restrict__ uy,
-    const T* __restrict__ ar, const T* __restrict__ dx, const T* __restrict__ dy,
-    T* __restrict__ fx, T* __restrict__ fy) {
+    const Dom& d, int i0, int j0, const S* __restrict__ qx, const S* __restrict__ qy,
+    const S* __restrict__ cx, const S* __restrict__ cy, const S* __restrict__ ax,
+    const S* __restrict__ ay, const S* __restrict__ ux, const S* __restrict__ uy,
+    const S* __restrict__ ar, const S* __restrict__ dx, const S* __restrict__ dy,
+    S* __restrict__ fx, S* __restrict__ fy) {
   const int lane = threadIdx.x;
   const int i = i0 - 3 + lane;  // column of this lane
   const int ni = d.ni, nj = d.nj;
@@ -249,7 +252,7 @@ __device__ __forceinline__ void fvtp2d_strip(
   // feeds only interfaces beyond the compute domain (every flux kept reads
   // cells inside the storage), so what such loads return is never used.
   const int ic = min(max(i, 0), ni - 1);
-  const uint32_t icb = (uint32_t)ic * (uint32_t)sizeof(T);  // byte offset of the column
+  const uint32_t icb = (uint32_t)ic * (uint32_t)sizeof(S);  // byte offset of the column
   // The column offset is passed through an empty asm at each use so that the
   // compiler keeps (uniform row address) + (32-bit lane offset), which maps
   // to the scalar-base form of the global loads and stores, instead of
@@ -259,10 +262,10 @@ __device__ __forceinline__ void fvtp2d_strip(
     asm volatile("" : "+v"(o));
     return o;
   };
-  auto ld = [&](const T* __restrict__ p, int j, int64_t stride) -> T {
+  auto ld = [&](const S* __restrict__ p, int j, int64_t stride) -> T {
     const int jc = __builtin_amdgcn_readfirstlane(min(max(j, 0), nj - 1));
     const char* __restrict__ row = (const char*)(p + (int64_t)jc * stride);  // wave-uniform
-    return *(const T*)(row + lane_off());
+    return T(*(const S*)(row + lane_off()));
   };
   // Flux stores are buffer stores: lanes or steps that own no output give
   // an offset beyond the buffer's record range, which the hardware drops,
@@ -272,12 +275,13 @@ __device__ __forceinline__ void fvtp2d_strip(
   auto st = [&](__amdgpu_buffer_rsrc_t rs, int j, bool keep, T v) {
     const uint32_t off = keep ? icb : 0x80000000u;
     const int rowb = __builtin_amdgcn_readfirstlane((int)((int64_t)min(max(j, 0), nj - 1) * sj *
-                                                          (int64_t)sizeof(T)));
-    if constexpr (sizeof(T) == 8) {
+                                                          (int64_t)sizeof(S)));
+    const S sv = S(v);
+    if constexpr (sizeof(S) == 8) {
       typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), rs, off, rowb, 0);
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, sv), rs, off, rowb, 0);
     } else {
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, off, rowb, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, sv), rs, off, rowb, 0);
     }
   };
   ColumnPPM<T, ORD, EY> yin, yout;
@@ -285,7 +289,7 @@ __device__ __forceinline__ void fvtp2d_strip(
   T sym_prev = T(0); // y-inner mean of face r-3
   // tile-edge width rows are clamped into the storage the same way
   auto jrow = [&](int j) { return (int64_t)min(max(j, 0), nj - 1) * sj2; };
-  const T* dycol = dy + ic;
+  const S* dycol = dy + ic;
   // every value a march step uses is loaded by the step before it, so the
   // loads of one step are in flight while the previous step computes
   struct StepIn {
@@ -351,14 +355,14 @@ __device__ __forceinline__ void fvtp2d_strip(
   }
 }
 
-template <typename T, int ORD>
+template <typename S, typename T, int ORD>
 __global__ void __launch_bounds__(MW)
-    fvtp2d_kernel(Dom d, const T* __restrict__ q_x, const T* __restrict__ q_y,
-                  const T* __restrict__ crx, const T* __restrict__ cry,
-                  const T* __restrict__ xfx, const T* __restrict__ yfx,
-                  const T* __restrict__ xmf, const T* __restrict__ ymf,
-                  const T* __restrict__ area, const T* __restrict__ dxa,
-                  const T* __restrict__ dya, T* __restrict__ fx, T* __restrict__ fy,
+    fvtp2d_kernel(Dom d, const S* __restrict__ q_x, const S* __restrict__ q_y,
+                  const S* __restrict__ crx, const S* __restrict__ cry,
+                  const S* __restrict__ xfx, const S* __restrict__ yfx,
+                  const S* __restrict__ xmf, const S* __restrict__ ymf,
+                  const S* __restrict__ area, const S* __restrict__ dxa,
+                  const S* __restrict__ dya, S* __restrict__ fx, S* __restrict__ fy,
                   int nbi, int nbj) {
   const int per_plane = nbi * nbj;
   const int total = per_plane * d.nk * d.ntiles;
@@ -372,8 +376,8 @@ __global__ void __launch_bounds__(MW)
   const int j0 = d.jsc + bj * MTJ;
   const int64_t off3 = (int64_t)t * d.st + (int64_t)k * d.sk;
   const int64_t off2 = (int64_t)t * d.st2;
-  const T* ux = (xmf ? xmf : xfx) + off3;
-  const T* uy = (ymf ? ymf : yfx) + off3;
+  const S* ux = (xmf ? xmf : xfx) + off3;
+  const S* uy = (ymf ? ymf : yfx) + off3;
   // the tile-edge formulas of a direction are needed when the strip's
   // stencil reach (cells i0-4 .. i0+MTI+4, rows j0-4 .. j0+MTJ+4) touches
   // an edge region of that direction
@@ -381,7 +385,7 @@ __global__ void __launch_bounds__(MW)
   const bool ex = near(i0 - 4, i0 + MTI + 4, d.is) || near(i0 - 4, i0 + MTI + 4, d.ie);
   const bool ey = near(j0 - 4, j0 + MTJ + 4, d.js) || near(j0 - 4, j0 + MTJ + 4, d.je);
 #define FV3_TP_STRIP(EX_, EY_)                                                                  \
-  fvtp2d_strip<T, ORD, EX_, EY_>(d, i0, j0, q_x + off3, q_y + off3, crx + off3, cry + off3,   \
+  fvtp2d_strip<S, T, ORD, EX_, EY_>(d, i0, j0, q_x + off3, q_y + off3, crx + off3, cry + off3,   \
                                  xfx + off3, yfx + off3, ux, uy, area + off2, dxa + off2,     \
                                  dya + off2, fx + off3, fy + off3)
   if (ex && ey) {
@@ -396,7 +400,8 @@ __global__ void __launch_bounds__(MW)
 #undef FV3_TP_STRIP
 }
 
-template <typename T>
+// T: storage type; C: arithmetic type of the kernel.
+template <typename T, typename C = T>
 static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T* q_y,
                          const T* crx, const T* cry, const T* xfx, const T* yfx,
                          const T* xmf, const T* ymf, const T* area, const T* dxa,
@@ -431,19 +436,19 @@ static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T*
                         d.nk * d.ntiles);
   switch (hord) {
     case 5:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, 5>), grid, block, 0, s, d, q_x, q_y, crx, cry,
+      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 5>), grid, block, 0, s, d, q_x, q_y, crx, cry,
                          xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
       break;
     case 6:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, 6>), grid, block, 0, s, d, q_x, q_y, crx, cry,
+      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 6>), grid, block, 0, s, d, q_x, q_y, crx, cry,
                          xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
       break;
     case 7:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, 7>), grid, block, 0, s, d, q_x, q_y, crx, cry,
+      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 7>), grid, block, 0, s, d, q_x, q_y, crx, cry,
                          xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
       break;
     default:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, 8>), grid, block, 0, s, d, q_x, q_y, crx, cry,
+      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 8>), grid, block, 0, s, d, q_x, q_y, crx, cry,
                          xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
       break;
   }
@@ -680,6 +685,17 @@ int32_t fv3_fvtp2d_f32(const fv3_domain* d, int32_t hord, const float* q_x, cons
   return fv3::fvtp2d_launch<float>(d, hord, q_x, q_y, crx, cry, x_area_flux, y_area_flux,
                                    x_mass_flux, y_mass_flux, area, dxa, dya, q_x_flux,
                                    q_y_flux, stream);
+}
+
+int32_t fv3_fvtp2d_mixed_f64(const fv3_domain* d, int32_t hord, const double* q_x,
+                             const double* q_y, const double* crx, const double* cry,
+                             const double* x_area_flux, const double* y_area_flux,
+                             const double* x_mass_flux, const double* y_mass_flux,
+                             const double* area, const double* dxa, const double* dya,
+                             double* q_x_flux, double* q_y_flux, void* stream) {
+  return fv3::fvtp2d_launch<double, float>(d, hord, q_x, q_y, crx, cry, x_area_flux,
+                                           y_area_flux, x_mass_flux, y_mass_flux, area, dxa,
+                                           dya, q_x_flux, q_y_flux, stream);
 }
 
 int32_t fv3_ppm_flux_f64(const fv3_domain* d, int32_t direction, int32_t ord, int32_t oi,

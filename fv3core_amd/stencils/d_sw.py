@@ -44,7 +44,8 @@ class _Params(ctypes.Structure):
         ("hord_vt", ctypes.c_int32), ("hydrostatic", ctypes.c_int32),
         ("nord_v4", ctypes.c_int32 * 4), ("nord_w4", ctypes.c_int32 * 4),
         ("nord_t4", ctypes.c_int32 * 4), ("div_k0", ctypes.c_int32),
-        ("div_nord", ctypes.c_int32), ("dddmp", ctypes.c_double), ("da_min_c", ctypes.c_double),
+        ("div_nord", ctypes.c_int32), ("transport_fp32", ctypes.c_int32),
+        ("dddmp", ctypes.c_double), ("da_min_c", ctypes.c_double),
         ("dd8", ctypes.c_double), ("d_con", ctypes.c_double),
         ("damp_vt_deln", ctypes.c_void_p), ("damp_t_deln", ctypes.c_void_p),
         ("damp_w_nosg", ctypes.c_void_p), ("damp_v_nosg", ctypes.c_void_p),
@@ -171,6 +172,8 @@ class DGridShallowWaterLagrangianDynamics:
         self._params = p
         self._dev_cols = {}
         self._work = Workspace(22)
+        # mixed precision (extension 1): FP32 arithmetic in the fv_tp_2d calls
+        self.transport_fp32 = False
 
     def _columns(self, device):
         key = str(device)
@@ -196,6 +199,7 @@ class DGridShallowWaterLagrangianDynamics:
         d.nk = self._nk
         cols = self._columns(delp.device)
         p = self._params
+        p.transport_fp32 = int(bool(self.transport_fp32))
         for k, t in cols.items():
             setattr(p, k, t.data_ptr())
         st = _State()

@@ -66,7 +66,8 @@ class AcousticDynamics:
 
     def __init__(self, comm, stencil_factory: StencilFactory, grid_data: GridData,
                  damping_coefficients: DampingCoefficients, grid_type, nested, stretched_grid,
-                 config: AcousticDynamicsConfig, ak, bk, pfull, phis, use_graph: bool = False):
+                 config: AcousticDynamicsConfig, ak, bk, pfull, phis, use_graph: bool = False,
+                 mixed_precision: bool = False):
         gi = stencil_factory.grid_indexing
         self._use_graph = bool(use_graph)
         self._graph = None
@@ -128,6 +129,9 @@ class AcousticDynamics:
         self.dgrid_shallow_water_lagrangian_dynamics = DGridShallowWaterLagrangianDynamics(
             stencil_factory, grid_data, damping_coefficients, column_namelist, nested,
             stretched_grid, dsw_cfg)
+        # mixed precision (north-star extension 1): FP32 arithmetic in d_sw's
+        # fv_tp_2d transports; storages and every other kernel stay FP64
+        self.dgrid_shallow_water_lagrangian_dynamics.transport_fp32 = bool(mixed_precision)
         self.cgrid_shallow_water_lagrangian_dynamics = CGridShallowWaterDynamics(
             stencil_factory, grid_data, nested, config.grid_type, config.nord)
         self.update_geopotential_height_on_c_grid = UpdateGeopotentialHeightOnCGrid(

@@ -99,6 +99,16 @@ int32_t fv3_fvtp2d_f32(const fv3_domain* d, int32_t hord, const float* q_x,
                        const float* x_mass_flux, const float* y_mass_flux,
                        const float* area, const float* dxa, const float* dya,
                        float* q_x_flux, float* q_y_flux, void* stream);
+/* Mixed-precision form (north-star extension 1; no counterpart in the
+ * reference, which computes in FP64 only): FP64 storages, as fv3_fvtp2d_f64,
+ * with the PPM and cross-update arithmetic of the kernel in FP32 (inputs
+ * rounded to FP32 on load, fluxes widened to FP64 on store). */
+int32_t fv3_fvtp2d_mixed_f64(const fv3_domain* d, int32_t hord, const double* q_x,
+                             const double* q_y, const double* crx, const double* cry,
+                             const double* x_area_flux, const double* y_area_flux,
+                             const double* x_mass_flux, const double* y_mass_flux,
+                             const double* area, const double* dxa, const double* dya,
+                             double* q_x_flux, double* q_y_flux, void* stream);
 
 /*
  * Tangent-linear model and adjoint of fv3_fvtp2d with respect to the
@@ -321,6 +331,8 @@ typedef struct fv3_dsw_params {
   int32_t nord_v4[4], nord_w4[4], nord_t4[4];  /* nord_v/w/t for k = 0, 1, 2, >= 3 */
   int32_t div_k0;                              /* first level with nord > 0 */
   int32_t div_nord;                            /* divergence-damping order there */
+  int32_t transport_fp32;  /* nonzero: the fv_tp_2d calls use fv3_fvtp2d_mixed_f64
+                              (FP32 arithmetic on the FP64 storages; extension 1) */
   double dddmp, da_min_c;
   double dd8;                                  /* (da_min_c * d4_bg) ** (div_nord + 1) */
   double d_con;                                /* namelist d_con */
