@@ -58,6 +58,8 @@ _SIGNATURES = {
     "fv3_last_error": ([], ctypes.c_char_p),
     "fv3_device_count": ([], _I),
     "fv3_fvtp2d_workspace_slabs": ([], _I),
+    "fv3_map_single_tlm_workspace_slabs": ([], _I),
+    "fv3_map_single_adjoint_workspace_slabs": ([], _I),
 }
 
 

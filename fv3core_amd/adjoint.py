@@ -9,10 +9,14 @@ input state: the call is its own checkpoint.  ``TransportLinearization``
 also exposes the tangent-linear model (``fv3_fvtp2d_tlm``) for dot-product
 tests.
 
+``RemapLinearization`` / ``MapScalarFn`` do the same for the vertical remap
+map_scalar (MapSingle with kord 9) through ``fv3_map_single_adjoint``.
+
 ``PeriodicTransport`` steps a scalar on a doubly periodic tile with the
 flux-form update ``q += (fx(i) - fx(i+1) + fy(j) - fy(j+1)) / area`` (the
 ``q_adjust`` form of fv3core/stencils/tracer_2d_1l.py:108-117 with unit
-layer mass), and :func:`cost_and_gradient` evaluates the 4D-Var-style cost
+layer mass), optionally followed by a remap from fixed Lagrangian to fixed
+Eulerian interfaces, and :func:`cost_and_gradient` evaluates the 4D-Var-style cost
 ``J = 0.5 ||q_T - q_obs||^2`` over ``n`` steps and ``dJ/dq0`` by the reverse
 sweep.  Courant numbers, area fluxes and metric terms are held fixed.
 
@@ -23,7 +27,7 @@ autograd-side state is kept as row-major ``(K, J, I)`` tensors, whose
 from __future__ import annotations
 
 import ctypes
-from typing import Tuple
+from typing import Optional, Tuple
 
 import torch
 import torch.nn.functional as F
@@ -127,6 +131,101 @@ class TransportLinearization:
         return gqx, gqy
 
 
+class RemapLinearization:
+    """map_scalar (MapSingle with kord 9, mode ``iv``) forward, tangent-linear
+    and adjoint with respect to the remapped field, on the columns of the
+    compute domain of ``grid_indexing`` (km = its vertical extent).  pe1, pe2
+    (interfaces), qs (IJ, used by iv = -2) and qmin are held fixed."""
+
+    def __init__(self, grid_indexing, mode: int = 1, qmin: float = 0.0, kord: int = 9):
+        if abs(int(kord)) != 9:
+            raise NotImplementedError("the remap derivative exists for kord 9")
+        self.gi = grid_indexing
+        self.kord = int(kord)
+        self.mode = int(mode)
+        self.qmin = float(qmin)
+        self.km = grid_indexing.domain[2]
+        self.box = (grid_indexing.isc, grid_indexing.iec, grid_indexing.jsc, grid_indexing.jec)
+        self._work = {}
+
+    def _domain(self, q, pe1, pe2, qs, *others):
+        if pe1.shape[-1] < self.km + 1 or pe2.shape[-1] < self.km + 1:
+            raise ValueError("pe1 and pe2 need km + 1 interface levels")
+        return _lib.make_domain(self.gi, q, self.km, qs, others3=(pe1, pe2) + others)
+
+    def _workspace(self, ref, nslabs: int):
+        ntiles, ni, nj, _, sj, sk, st = _lib.layout3(ref)
+        need = (ntiles - 1) * st + (self.km + 1) * sk
+        need = (need + 63) // 64 * 64
+        w = self._work.get(nslabs)
+        if w is None or w[1] < need or w[0].device != ref.device:
+            w = (torch.empty(nslabs * need, dtype=torch.float64, device=ref.device), need)
+            self._work[nslabs] = w
+        return w
+
+    def forward(self, q, pe1, pe2, qs=None):
+        """A remapped copy of q (columns outside the box and level km unchanged)."""
+        _lib.require_device(q, pe1, pe2, qs)
+        out = new_ijk(q, zero=False)
+        out.copy_(q)
+        d = self._domain(out, pe1, pe2, qs)
+        work, slab = self._workspace(out, 7)
+        i1, i2, j1, j2 = self.box
+        _lib.check(_lib.lib().fv3_map_single_f64(
+            ctypes.byref(d), self.kord, self.mode, i1, i2, j1, j2, _lib.ptr(out), _lib.ptr(pe1),
+            _lib.ptr(pe2), _lib.ptr(qs), ctypes.c_double(self.qmin), _lib.ptr(work),
+            ctypes.c_int64(slab), ctypes.c_void_p(_lib.stream_handle())))
+        return out
+
+    def tlm(self, q, dq, pe1, pe2, qs=None):
+        """(remapped q, its perturbation for the perturbation dq of q)."""
+        _lib.require_device(q, dq, pe1, pe2, qs)
+        out, dout = new_ijk(q, zero=False), new_ijk(q, zero=False)
+        out.copy_(q)
+        dout.copy_(dq)
+        d = self._domain(out, pe1, pe2, qs, dout)
+        work, slab = self._workspace(out, _lib.lib().fv3_map_single_tlm_workspace_slabs())
+        i1, i2, j1, j2 = self.box
+        _lib.check(_lib.lib().fv3_map_single_tlm_f64(
+            ctypes.byref(d), self.kord, self.mode, i1, i2, j1, j2, _lib.ptr(out), _lib.ptr(dout),
+            _lib.ptr(pe1), _lib.ptr(pe2), _lib.ptr(qs), ctypes.c_double(self.qmin),
+            _lib.ptr(work), ctypes.c_int64(slab), ctypes.c_void_p(_lib.stream_handle())))
+        return out, dout
+
+    def adjoint(self, q, g_out, pe1, pe2, qs=None):
+        """Cotangent of q for the cotangent g_out of the remapped field."""
+        _lib.require_device(q, g_out, pe1, pe2, qs)
+        g_in = new_ijk(q, zero=False)
+        g_in.copy_(g_out)  # identity outside the remapped columns and levels
+        d = self._domain(q, pe1, pe2, qs, g_out, g_in)
+        work, slab = self._workspace(q, _lib.lib().fv3_map_single_adjoint_workspace_slabs())
+        i1, i2, j1, j2 = self.box
+        _lib.check(_lib.lib().fv3_map_single_adjoint_f64(
+            ctypes.byref(d), self.kord, self.mode, i1, i2, j1, j2, _lib.ptr(q), _lib.ptr(pe1),
+            _lib.ptr(pe2), _lib.ptr(qs), ctypes.c_double(self.qmin), _lib.ptr(g_out),
+            _lib.ptr(g_in), _lib.ptr(work), ctypes.c_int64(slab),
+            ctypes.c_void_p(_lib.stream_handle())))
+        return g_in
+
+
+class MapScalarFn(torch.autograd.Function):
+    """q_out = map_scalar(q) with the hand-written adjoint as backward."""
+
+    @staticmethod
+    def forward(ctx, q, lin: RemapLinearization, pe1, pe2, qs):
+        q_s = as_ijk_storage(q)
+        out = lin.forward(q_s, pe1, pe2, qs)
+        ctx.save_for_backward(q_s)
+        ctx.lin = lin
+        ctx.aux = (pe1, pe2, qs)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (q,) = ctx.saved_tensors
+        return ctx.lin.adjoint(q, as_ijk_storage(g), *ctx.aux), None, None, None, None
+
+
 class FvTp2dFn(torch.autograd.Function):
     """(fx, fy) = fv_tp_2d(q_x, q_y) with the hand-written adjoint as backward.
 
@@ -165,7 +264,8 @@ class PeriodicTransport:
     every step.
     """
 
-    def __init__(self, grid_indexing, inputs: dict, hord: int, n_halo: int = 3):
+    def __init__(self, grid_indexing, inputs: dict, hord: int, n_halo: int = 3,
+                 remap: Optional[dict] = None):
         gi = grid_indexing
         self.gi = gi
         self.h = n_halo
@@ -182,6 +282,13 @@ class PeriodicTransport:
         h, n = self.h, self.n
         if n < max(h, ni - h - n, nj - h - n):
             raise ValueError("the tile is narrower than its halo")
+        # optional vertical remap after each transport step: dict(pe1, pe2,
+        # mode=1, qmin=0.0, qs=None) with (I, J, K) interface storages
+        self.remap = None
+        if remap is not None:
+            self.remap = (RemapLinearization(gi, remap.get("mode", 1), remap.get("qmin", 0.0)),
+                          as_ijk_storage(remap["pe1"]), as_ijk_storage(remap["pe2"]),
+                          remap.get("qs"))
         rarea = 1.0 / inputs["area"]                       # (I, J)
         self._rarea = rarea.t()[None, h:h + n, h:h + n]   # (1, J, I) interior
 
@@ -210,7 +317,11 @@ class PeriodicTransport:
             self._rarea
         ni, nj, _ = self.shape
         dq = F.pad(dq, (h, ni - h - n, h, nj - h - n))
-        return qf + dq
+        q_new = qf + dq
+        if self.remap is not None:
+            lin, pe1, pe2, qs = self.remap
+            q_new = MapScalarFn.apply(ijk_view(q_new), lin, pe1, pe2, qs).permute(2, 1, 0)
+        return q_new
 
     def run(self, q0_kji: torch.Tensor, nsteps: int) -> torch.Tensor:
         q = q0_kji

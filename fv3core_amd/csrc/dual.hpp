@@ -30,6 +30,8 @@ struct Dual {
 #pragma unroll
     for (int n = 0; n < N; ++n) d[n] = 0.0;
   }
+  // the value part, for code that stores flags as numbers and reads them back
+  __device__ __forceinline__ explicit operator int() const { return (int)v; }
   // value x with unit derivative in direction m
   __device__ __forceinline__ static Dual seed(double x, int m) {
     Dual r(x);
@@ -78,6 +80,16 @@ __device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, const Dual<N>& b)
 #pragma unroll
   for (int n = 0; n < N; ++n) r.d[n] = (a.d[n] - r.v * b.d[n]) / b.v;
   return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N>& operator+=(Dual<N>& a, const Dual<N>& b) {
+  a = a + b;
+  return a;
+}
+template <int N>
+__device__ __forceinline__ Dual<N>& operator-=(Dual<N>& a, const Dual<N>& b) {
+  a = a - b;
+  return a;
 }
 template <int N>
 __device__ __forceinline__ bool operator<(const Dual<N>& a, const Dual<N>& b) {

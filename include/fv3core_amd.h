@@ -202,6 +202,34 @@ int32_t fv3_map_single_f32(const fv3_domain* d, int32_t kord, int32_t mode, int3
                            int64_t work_slab, void* stream);
 
 /*
+ * Tangent-linear model and adjoint of fv3_map_single_f64 (map_scalar), kord 9,
+ * with respect to the remapped field, at the branch decisions of the forward;
+ * pe1, pe2, qs and qmin are held fixed.  No counterpart in the reference,
+ * which has no derivative code: they linearise MapSingle.__call__
+ * (fv3core/stencils/map_single.py:130-173) for the adjoint of the vertical
+ * remap.  Arguments as fv3_map_single.
+ *   tlm:     q1 is remapped in place as by fv3_map_single, and dq1 (its
+ *            perturbation) becomes the perturbation of the result.
+ *            work: fv3_map_single_tlm_workspace_slabs() slabs.
+ *   adjoint: q1 (the forward input, unchanged) and g_out (cotangent of the
+ *            remapped field) -> g_q1 on the box columns, levels [0, km)
+ *            (other entries are not written).  work:
+ *            fv3_map_single_adjoint_workspace_slabs() slabs.
+ * Slabs have work_slab elements, work_slab >= (ntiles-1)*st + (km+1)*sk.
+ */
+int32_t fv3_map_single_tlm_workspace_slabs(void);
+int32_t fv3_map_single_adjoint_workspace_slabs(void);
+int32_t fv3_map_single_tlm_f64(const fv3_domain* d, int32_t kord, int32_t mode, int32_t i1,
+                               int32_t i2, int32_t j1, int32_t j2, double* q1, double* dq1,
+                               const double* pe1, const double* pe2, const double* qs,
+                               double qmin, double* work, int64_t work_slab, void* stream);
+int32_t fv3_map_single_adjoint_f64(const fv3_domain* d, int32_t kord, int32_t mode, int32_t i1,
+                                   int32_t i2, int32_t j1, int32_t j2, const double* q1,
+                                   const double* pe1, const double* pe2, const double* qs,
+                                   double qmin, const double* g_out, double* g_q1,
+                                   double* work, int64_t work_slab, void* stream);
+
+/*
  * RiemannSolverC.__call__ (fv3core/stencils/riem_solver_c.py:119-165):
  * precompute + sim1_solver + finalize on [isc-1, iec+1] x [jsc-1, jec+1],
  * d->nk = nz + 1 levels, FP64.  Outputs gz and pef; w3 is read only.

@@ -244,3 +244,127 @@ def test_gpu_multistep_cost_gradient(hord):
         print(f"hord={hord} dir {seed}: J={cost:.6e} FD {fd:.10e} adjoint {ad:.10e} "
               f"rel {_rel(fd, ad):.1e}")
         assert _rel(fd, ad) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# vertical remap (map_scalar, kord 9)
+# ---------------------------------------------------------------------------
+
+
+def _remap_setup(n, nz, mode, workload=4):
+    from fv3core_amd.adjoint import RemapLinearization
+    from fv3core_amd.workloads import remap_inputs
+    d = remap_inputs(n, nz, workload=workload)
+    gi = GridIndexing((n, n, nz), N_HALO, True, True, True, True)
+    dev = {k: _dev(d[k]) for k in ("pe1", "pe2", "pt", "q", "w", "qs")}
+    return d, dev, RemapLinearization(gi, mode=mode, qmin=0.0)
+
+
+def _remap_field(dev, mode):
+    # signed field for the iv modes that allow negative values, a positive one
+    # otherwise; both noisy, so no two neighbouring means tie (a tie makes the
+    # kord-9 extremum flag gc(k) gc(k+1) < 0 flip under any perturbation,
+    # which finite differences would see as a jump)
+    return dev["w"] if mode in (-2, 1, 2) else dev["pt"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [-2, -1, 0, 1, 2])
+def test_gpu_remap_tlm_homogeneity(mode):
+    """kord 9 with qmin = 0 is positively homogeneous in q: TLM(q; q) = remap(q)."""
+    n, nz = 16, 79
+    d, dev, lin = _remap_setup(n, nz, mode)
+    q = _ijk(_remap_field(dev, mode))
+    qs = dev["qs"] * 0.0 if mode == -2 else None  # qs enters affinely; zero keeps homogeneity
+    out = lin.forward(q, dev["pe1"], dev["pe2"], qs)
+    out2, dout = lin.tlm(q, q, dev["pe1"], dev["pe2"], qs)
+    torch.cuda.synchronize()
+    h = N_HALO
+    a = out[h:h + n, h:h + n, :nz].cpu().numpy()
+    b = dout[h:h + n, h:h + n, :nz].cpu().numpy()
+    assert np.array_equal(out.cpu().numpy(), out2.cpu().numpy())
+    err = np.abs(a - b).max() / np.abs(a).max()
+    print(f"mode={mode}: max |TLM(q;q) - remap(q)| / max|remap| = {err:.2e}")
+    assert err < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [-2, -1, 0, 1, 2])
+def test_gpu_remap_dot_product(mode):
+    """The hand-written reverse sweep against the Dual<1> TLM of the forward kernel."""
+    n, nz = 16, 79
+    d, dev, lin = _remap_setup(n, nz, mode)
+    q = _ijk(_remap_field(dev, mode))
+    g = torch.Generator(device="cpu").manual_seed(40 + mode)
+    dq = _ijk(torch.randn(q.shape, generator=g, dtype=q.dtype).cuda())
+    gout = _ijk(torch.randn(q.shape, generator=g, dtype=q.dtype).cuda())
+    qs = dev["qs"] if mode == -2 else None
+    _, dout = lin.tlm(q, dq, dev["pe1"], dev["pe2"], qs)
+    gin = lin.adjoint(q, gout, dev["pe1"], dev["pe2"], qs)
+    lhs = float((dout * gout).sum())
+    rhs = float((dq * gin).sum())
+    print(f"mode={mode}: <L dq, g> = {lhs:.15e}, <dq, L^T g> = {rhs:.15e}, "
+          f"rel {_rel(lhs, rhs):.1e}")
+    assert _rel(lhs, rhs) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gpu_remap_tlm_finite_differences(mode):
+    n, nz = 16, 79
+    d, dev, lin = _remap_setup(n, nz, mode)
+    q = _ijk(_remap_field(dev, mode))
+    g = torch.Generator(device="cpu").manual_seed(8)
+    dq = _ijk(torch.randn(q.shape, generator=g, dtype=q.dtype).cuda())
+    gout = torch.randn(q.shape, generator=g, dtype=q.dtype).cuda()
+    # a small step, so that (almost) no column crosses a limiter kink: the
+    # kord-9 selections switch where two O(1)-spread quantities tie
+    eps = 1e-9 * float(q.abs().max())
+    fp = lin.forward(_ijk(q + eps * dq), dev["pe1"], dev["pe2"])
+    fm = lin.forward(_ijk(q - eps * dq), dev["pe1"], dev["pe2"])
+    fd = float(((fp - fm) * gout).sum()) / (2 * eps)
+    _, dout = lin.tlm(q, dq, dev["pe1"], dev["pe2"])
+    tl = float((dout * gout).sum())
+    print(f"mode={mode}: FD {fd:.12e} TLM {tl:.12e} rel {_rel(fd, tl):.1e}")
+    assert _rel(fd, tl) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hord", [8])
+def test_gpu_multistep_transport_remap_gradient(hord):
+    """8 steps of fv_tp_2d + map_scalar (kord 9): reverse sweep vs central FD of J."""
+    from fv3core_amd.adjoint import PeriodicTransport, cost_and_gradient
+    from fv3core_amd.workloads import remap_inputs
+    n, nz, nsteps = 24, 79, 8
+    d = _periodic(n, nz)
+    r = remap_inputs(n, nz, workload=5)
+    gi = GridIndexing((n, n, nz), N_HALO, False, False, False, False)
+    dev = {k: _dev(v) for k, v in d.items()}
+    remap = dict(pe1=_dev(r["pe1"]), pe2=_dev(r["pe2"]), mode=1, qmin=0.0)
+    model = PeriodicTransport(gi, dev, hord, remap=remap)
+    q0 = torch.as_tensor(d["q"]).permute(2, 1, 0).contiguous().cuda()
+    g = torch.Generator(device="cpu").manual_seed(13)
+    with torch.no_grad():
+        q_obs = model.run(q0 + 1e-2 * torch.randn(q0.shape, generator=g, dtype=q0.dtype).cuda(),
+                          nsteps)
+    cost, grad = cost_and_gradient(model, q0, q_obs, nsteps)
+    h = N_HALO
+    for seed in (31, 32):
+        gd = torch.Generator(device="cpu").manual_seed(seed)
+        dq = torch.randn(q0.shape, generator=gd, dtype=q0.dtype).cuda()
+        dq_int = torch.zeros_like(dq)
+        dq_int[:, h:h + n, h:h + n] = dq[:, h:h + n, h:h + n]
+        # kord 9 flattens a layer where two adjacent means tie, a jump in the
+        # map: the step is kept small enough that no level pair of the 8
+        # steps' ~45k crosses a tie (1e-6 crosses about one)
+        eps = 1e-9
+        with torch.no_grad():
+            jp = 0.5 * float(((model.interior(model.run(q0 + eps * dq_int, nsteps)) -
+                               model.interior(q_obs)) ** 2).sum())
+            jm = 0.5 * float(((model.interior(model.run(q0 - eps * dq_int, nsteps)) -
+                               model.interior(q_obs)) ** 2).sum())
+        fd = (jp - jm) / (2 * eps)
+        ad = float((grad * dq_int).sum())
+        print(f"hord={hord} dir {seed}: J={cost:.6e} FD {fd:.10e} adjoint {ad:.10e} "
+              f"rel {_rel(fd, ad):.1e}")
+        assert _rel(fd, ad) < 1e-5
