@@ -107,21 +107,34 @@ template <int N>
 __device__ __forceinline__ bool operator>=(const Dual<N>& a, const Dual<N>& b) {
   return a.v >= b.v;
 }
+// c ? a : b, component by component (a conditional expression on two
+// structures would be lowered through memory)
+template <int N>
+__device__ __forceinline__ Dual<N> pick(bool c, const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r;
+  r.v = c ? a.v : b.v;
+#pragma unroll
+  for (int n = 0; n < N; ++n) r.d[n] = c ? a.d[n] : b.d[n];
+  return r;
+}
 template <int N>
 __device__ __forceinline__ Dual<N> fabs(const Dual<N>& a) {
-  Dual<N> r = a.v < 0.0 ? -a : a;
+  const bool neg = a.v < 0.0;
+  Dual<N> r;
   r.v = ::fabs(a.v);  // the value is the double fabs, signed zeros included
+#pragma unroll
+  for (int n = 0; n < N; ++n) r.d[n] = neg ? -a.d[n] : a.d[n];
   return r;
 }
 // fmin/fmax of IEEE doubles return the other operand when one is NaN; the
 // stencils never pass NaN, so the comparison form is equivalent here.
 template <int N>
 __device__ __forceinline__ Dual<N> fmin(const Dual<N>& a, const Dual<N>& b) {
-  return b.v < a.v ? b : a;
+  return pick(b.v < a.v, b, a);
 }
 template <int N>
 __device__ __forceinline__ Dual<N> fmax(const Dual<N>& a, const Dual<N>& b) {
-  return b.v > a.v ? b : a;
+  return pick(b.v > a.v, b, a);
 }
 
 }  // namespace fv3
