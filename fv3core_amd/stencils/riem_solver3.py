@@ -24,7 +24,7 @@ class RiemannSolver3:
         self._grid_indexing = stencil_factory.grid_indexing
         self._config = config
         self._nk = self._grid_indexing.domain[2] + 1
-        self._work = Workspace(8)
+        self._work = Workspace(3)
 
     def __call__(self, last_call: bool, dt: float, cappa, ptop: float, zs, wsd, delz, q_con,
                  delp, pt, zh, pe, ppe, pk3, pk, peln, w):

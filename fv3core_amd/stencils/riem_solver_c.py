@@ -21,7 +21,7 @@ class RiemannSolverC:
         self._grid_indexing = stencil_factory.grid_indexing
         self._p_fac = float(p_fac)
         self._nk = self._grid_indexing.domain[2] + 1
-        self._work = Workspace(10)
+        self._work = Workspace(3)
 
     def __call__(self, dt2: float, cappa, ptop: float, hs, ws, ptc, q_con, delpc, gz, pef, w3):
         """gz, pef <- nonhydrostatic C-grid solve (w3 is read only)."""

@@ -232,8 +232,8 @@ int32_t fv3_map_single_adjoint_f64(const fv3_domain* d, int32_t kord, int32_t mo
 /*
  * RiemannSolverC.__call__ (fv3core/stencils/riem_solver_c.py:119-165):
  * precompute + sim1_solver + finalize on [isc-1, iec+1] x [jsc-1, jec+1],
- * d->nk = nz + 1 levels, FP64.  Outputs gz and pef; w3 is read only.
- * work: 10 slabs of work_slab elements (>= (ntiles-1)*st + nk*sk).
+ * d->nk = nz + 1 levels (4 .. 128), FP64.  Outputs gz and pef; w3 is read
+ * only.  work: 3 slabs of work_slab elements (>= (ntiles-1)*st + nk*sk).
  */
 int32_t fv3_riem_solver_c_f64(const fv3_domain* d, double dt2, double ptop, double p_fac,
                               const double* cappa, const double* hs, const double* ws,
@@ -243,8 +243,9 @@ int32_t fv3_riem_solver_c_f64(const fv3_domain* d, double dt2, double ptop, doub
 
 /*
  * RiemannSolver3.__call__ (fv3core/stencils/riem_solver3.py:145-242) on the
- * compute domain, d->nk = nz + 1 levels, FP64 (a_imp > 0.999 path).
- * In/out: delz, zh, pe, ppe, pk3, pk, peln, w.  work: 8 slabs.
+ * compute domain, d->nk = nz + 1 levels (4 .. 128), FP64 (a_imp > 0.999
+ * path).  In/out: delz, zh, pe, ppe, pk3, pk, peln, w; pe is written only
+ * on the last call.  work: 3 slabs.
  */
 int32_t fv3_riem_solver3_f64(const fv3_domain* d, int32_t last_call, double dt, double ptop,
                              double p_fac, int32_t use_logp, double beta, const double* cappa,
