@@ -1,10 +1,11 @@
 // AGrid2BGridFourthOrder (a2b_ord4): cell-centre values to cell corners.
 //
-// Reference: fv3core/stencils/a2b_ord4.py:1-695.  Three launches:
-//   1 edges:  the four corner extrapolations (great-circle distances on
-//             the A/B-grid longitudes and latitudes) and the tile-edge
-//             lines qout_x_edge / qout_y_edge, into qout and the edge
-//             temporary;
+// Reference: fv3core/stencils/a2b_ord4.py:1-695.  Four launches:
+//   0 weights: the corner extrapolation weights from great-circle
+//             distances on the A/B-grid longitudes and latitudes (12 per
+//             tile, grid only);
+//   1 edges:  the four corner extrapolations and the tile-edge lines
+//             qout_x_edge / qout_y_edge, into qout and the edge temporary;
 //   2 means:  ppm_volume_mean_x on [isc, iec+1] x [jsc-2, jec+2] and
 //             ppm_volume_mean_y on [isc-2, iec+2] x [jsc, jec+1];
 //   3 interp: a2b_interpolation on the corners off the tile-edge lines
@@ -64,17 +65,35 @@ __device__ __forceinline__ T gc_dist(T p1a, T p1b, T p2a, T p2b) {
   return asin(sqrt(tb + cos(p1b) * cos(p2b) * ta)) * T(2);
 }
 
-// extrap_corner (a2b_ord4.py:44-57) from the corner point o2c and the two
-// centres at (di1, dj1), (di2, dj2) relative to the point.
+// The three extrapolations of each cube corner (a2b_ord4.py:617-658), as
+// centre offsets (di1, dj1, di2, dj2) from the corner point; corners in the
+// order (isc, jsc), (iec+1, jsc), (iec+1, jec+1), (isc, jec+1).
+__constant__ int kCornerTaps[4][3][4] = {
+    {{0, 0, 1, 1}, {-1, 0, -2, 1}, {0, -1, 1, -2}},
+    {{-1, 0, -2, 1}, {-1, -1, -2, -2}, {0, 0, 1, 1}},
+    {{-1, -1, -2, -2}, {0, -1, 1, -2}, {-1, 0, -2, 1}},
+    {{0, -1, 1, -2}, {-1, -1, -2, -2}, {0, 0, 1, 1}},
+};
+
+// Weight x1 / (x2 - x1) of extrapolation e (corner e / 3, tap e % 3) of
+// tile blockIdx.x: extrap_corner is qa + x1 / (x2 - x1) (qa - qb) with
+// great-circle distances that depend on the grid only, so they are formed
+// once per call here instead of once per level in the perimeter launch.
 template <typename T>
-__device__ __forceinline__ T extrap(const Met<T>& m, const Dom& d, const T* q, int64_t o,
-                                    int64_t o2, int di1, int dj1, int di2, int dj2) {
-  const int64_t a1 = o2 + di1 + (int64_t)dj1 * d.sj2, a2 = o2 + di2 + (int64_t)dj2 * d.sj2;
+__global__ void __launch_bounds__(64) corner_weights_kernel(Met<T> m, Dom d, T* w) {
+  const int e = threadIdx.x;
+  if (e >= 12) return;
+  const int t = blockIdx.x;
+  const int corner = e / 3, tap = e % 3;
+  const int ic = (corner == 1 || corner == 2) ? d.iec + 1 : d.isc;
+  const int jc = (corner >= 2) ? d.jec + 1 : d.jsc;
+  const int64_t o2 = at2(d, ic, jc, t);
+  const int* tp = kCornerTaps[corner][tap];
+  const int64_t a1 = o2 + tp[0] + (int64_t)tp[1] * d.sj2, a2 = o2 + tp[2] + (int64_t)tp[3] * d.sj2;
   const T p0a = m.bgrid1[o2], p0b = m.bgrid2[o2];
   const T x1 = gc_dist(m.agrid1[a1], m.agrid2[a1], p0a, p0b);
   const T x2 = gc_dist(m.agrid1[a2], m.agrid2[a2], p0a, p0b);
-  const T qa = q[o + di1 + (int64_t)dj1 * d.sj], qb = q[o + di2 + (int64_t)dj2 * d.sj];
-  return qa + x1 / (x2 - x1) * (qa - qb);
+  w[t * 12 + e] = x1 / (x2 - x1);
 }
 
 template <typename T>
@@ -141,6 +160,12 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
   const bool so = has_edge(dom->j_start), no = has_edge(dom->j_end);
   const int js2 = so ? d.jsc + 1 : d.jsc, je1 = no ? d.jec : d.jec + 1;
   const int is2 = w ? d.isc + 1 : d.isc, ie1 = e ? d.iec : d.iec + 1;
+  // 0: corner extrapolation weights, 12 per tile, into the head of the qx
+  // slab (launch 2 writes qx only after launch 1 has read them)
+  T* const cw = qx;
+  hipLaunchKernelGGL((corner_weights_kernel<T>), dim3((unsigned)d.ntiles), dim3(64), 0, s, m, d,
+                     cw);
+  FV3_HIP_CHECK_LAUNCH(who);
   // 1: corners and edge lines: only the perimeter of [isc, iec+1]^2 can hold them
   const Box bx{d.isc, d.iec + 1, d.jsc, d.jec + 1};
   launch_perimeter(s, d, bx, k0, nk, [=] __device__(int i, int j, int k, int t) {
@@ -151,25 +176,16 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
     if (ci && cj) {
       // _sw_corner at (isc, jsc), _nw_corner at (iec+1, jsc), _ne_corner at
       // (iec+1, jec+1), _se_corner at (isc, jec+1) (a2b_ord4.py:617-658)
-      T e1, e2, e3;
-      if (i == d.isc && j == d.jsc) {
-        e1 = extrap(m, d, qin, o, o2, 0, 0, 1, 1);
-        e2 = extrap(m, d, qin, o, o2, -1, 0, -2, 1);
-        e3 = extrap(m, d, qin, o, o2, 0, -1, 1, -2);
-      } else if (j == d.jsc) {
-        e1 = extrap(m, d, qin, o, o2, -1, 0, -2, 1);
-        e2 = extrap(m, d, qin, o, o2, -1, -1, -2, -2);
-        e3 = extrap(m, d, qin, o, o2, 0, 0, 1, 1);
-      } else if (i == d.iec + 1) {
-        e1 = extrap(m, d, qin, o, o2, -1, -1, -2, -2);
-        e2 = extrap(m, d, qin, o, o2, 0, -1, 1, -2);
-        e3 = extrap(m, d, qin, o, o2, -1, 0, -2, 1);
-      } else {
-        e1 = extrap(m, d, qin, o, o2, 0, -1, 1, -2);
-        e2 = extrap(m, d, qin, o, o2, -1, -1, -2, -2);
-        e3 = extrap(m, d, qin, o, o2, 0, 0, 1, 1);
+      const int corner = j == d.jsc ? (i == d.isc ? 0 : 1) : (i == d.iec + 1 ? 2 : 3);
+      T ex[3];
+#pragma unroll
+      for (int tap = 0; tap < 3; ++tap) {
+        const int* tp = kCornerTaps[corner][tap];
+        const T qa = qin[o + tp[0] + (int64_t)tp[1] * d.sj];
+        const T qb = qin[o + tp[2] + (int64_t)tp[3] * d.sj];
+        ex[tap] = qa + cw[t * 12 + corner * 3 + tap] * (qa - qb);
       }
-      val = (e1 + e2 + e3) * (T(1) / T(3));
+      val = (ex[0] + ex[1] + ex[2]) * (T(1) / T(3));
     } else if (ci && ((i == d.isc && w) || (i == d.iec + 1 && e)) && j >= js2 && j <= je1) {
       // qout_x_edge (a2b_ord4.py:313-323): q2 at j and j - 1
       const T* dxa = m.dxa;
