@@ -6,10 +6,11 @@
 //             tile, grid only);
 //   1 edges:  the four corner extrapolations and the tile-edge lines
 //             qout_x_edge / qout_y_edge, into qout and the edge temporary;
-//   2 means:  ppm_volume_mean_x on [isc, iec+1] x [jsc-2, jec+2] and
-//             ppm_volume_mean_y on [isc-2, iec+2] x [jsc, jec+1];
-//   3 interp: a2b_interpolation on the corners off the tile-edge lines
-//             (and the optional copy back into qin).
+//   2 means + interp: ppm_volume_mean_x on [isc, iec+1] x [jsc-2, jec+2]
+//             and ppm_volume_mean_y on [isc-2, iec+2] x [jsc, jec+1], each
+//             formed in LDS only where a block's corners read it, then
+//             a2b_interpolation on the corners off the tile-edge lines;
+//   3 copy:   the optional copy back into qin.
 #include "kernels.hpp"
 
 namespace fv3 {
@@ -148,6 +149,109 @@ __device__ __forceinline__ T volume_mean(const T (&q7)[7], const T (&dx7)[7], in
   return T(kB2) * (q[-2] + q[1]) + T(kB1) * (q[-1] + q[0]);
 }
 
+// Launches 2 and 3 fused: one workgroup owns a TI x TJ block of corners of
+// one level.  It stages q over the block plus a 3-cell halo in LDS, forms
+// the x volume means on the rows j-3 .. j+2 and the y volume means on the
+// columns i-3 .. i+2 of its corners there, then interpolates.  Lane x of
+// every wave owns LDS column x; wave y owns rows y, y+4, ...  (The metric
+// terms are read from global memory: staging them in LDS as well, once per
+// block over several levels, measured slower through the lower occupancy.)
+constexpr int AB_W = 64;           // LDS columns = TI + 6
+constexpr int AB_TI = AB_W - 6;    // corners per block in i
+constexpr int AB_TJ = 16;          // corners per block in j
+constexpr int AB_H = AB_TJ + 6;    // q rows: j0-3 .. j0+TJ+2
+constexpr int AB_NY = 4;           // waves per block
+
+struct A2BArgs {
+  int ci0, cj0, nci, ncj;  // corner box (interpolation off the edge lines)
+  int nbi;                 // blocks along i
+  int k0;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(AB_W * AB_NY)
+    a2b_interp_kernel(Dom d, Met<T> m, A2BArgs a, const T* __restrict__ qin,
+                      const T* __restrict__ edges, T* __restrict__ qout) {
+  __shared__ T sq[AB_H][AB_W];
+  __shared__ T sx[AB_H - 1][AB_W];  // x means on rows j0-3 .. j0+TJ+1
+  __shared__ T sy[AB_TJ][AB_W];     // y means on rows j0 .. j0+TJ-1
+  const int li = threadIdx.x, ty = threadIdx.y;
+  const int bi = blockIdx.x % a.nbi, bj = blockIdx.x / a.nbi;
+  const int k = a.k0 + (int)blockIdx.y, t = blockIdx.z;
+  const int i0 = a.ci0 + bi * AB_TI, j0 = a.cj0 + bj * AB_TJ;
+  const int i = i0 - 3 + li;  // absolute column of this lane
+  const int jb = j0 - 3;      // absolute row of LDS row 0
+  const bool col_ok = i >= 0 && i < d.ni;
+  const int64_t off3 = (int64_t)t * d.st + (int64_t)k * d.sk;
+  const int64_t off2 = (int64_t)t * d.st2;
+  for (int r = ty; r < AB_H; r += AB_NY) {
+    const int j = jb + r;
+    sq[r][li] = (col_ok && j >= 0 && j < d.nj) ? qin[off3 + (int64_t)j * d.sj + i] : T(0);
+  }
+  __syncthreads();
+  // x means where launch 2 formed them: i in [isc, iec+1], j in [jsc-2, jec+2]
+  if (li >= 3 && li < AB_W - 3 && i >= d.isc && i <= d.iec + 1) {
+    for (int r = ty; r < AB_H - 1; r += AB_NY) {
+      const int j = jb + r;
+      if (j >= d.jsc - 2 && j <= d.jec + 2) {
+        const int64_t o2 = off2 + (int64_t)j * d.sj2 + i;
+        T q7[7], dx7[7];
+#pragma unroll
+        for (int n = 0; n < 7; ++n) {
+          q7[n] = sq[r][li + n - 3];
+          dx7[n] = m.dxa[o2 + n - 3];
+        }
+        sx[r][li] = volume_mean(q7, dx7, i, d.is, d.ie);
+      }
+    }
+  }
+  // y means where launch 2 formed them: i in [isc-2, iec+2], j in [jsc, jec+1]
+  if (li < AB_W - 1 && i >= d.isc - 2 && i <= d.iec + 2) {
+    for (int r = ty; r < AB_TJ; r += AB_NY) {
+      const int j = j0 + r;
+      if (j >= d.jsc && j <= d.jec + 1) {
+        const int64_t o2 = off2 + (int64_t)j * d.sj2 + i;
+        T q7[7], dy7[7];
+#pragma unroll
+        for (int n = 0; n < 7; ++n) {
+          q7[n] = sq[r + n][li];
+          dy7[n] = m.dya[o2 + (int64_t)(n - 3) * d.sj2];
+        }
+        sy[r][li] = volume_mean(q7, dy7, j, d.js, d.je);
+      }
+    }
+  }
+  __syncthreads();
+  // a2b_interpolation on the block's corners (qx rows and qy columns -3 .. +2)
+  if (li < 3 || li >= 3 + AB_TI || i >= a.ci0 + a.nci) return;
+  const int64_t sj = d.sj;
+  for (int r = ty; r < AB_TJ; r += AB_NY) {
+    const int j = j0 + r;
+    if (j >= a.cj0 + a.ncj) break;
+    const int x = r + 3;  // sx row of j
+    const int64_t o = off3 + (int64_t)j * sj + i;
+    T qxx = T(kA2) * (sx[x - 2][li] + sx[x + 1][li]) + T(kA1) * (sx[x - 1][li] + sx[x][li]);
+    T qyy = T(kA2) * (sy[r][li - 2] + sy[r][li + 1]) + T(kA1) * (sy[r][li - 1] + sy[r][li]);
+    if (j == d.js + 1) {
+      const T up = T(kA2) * (sx[x - 1][li] + sx[x + 2][li]) + T(kA1) * (sx[x][li] + sx[x + 1][li]);
+      qxx = T(kCc1) * (sx[x - 1][li] + sx[x][li]) + T(kCc2) * (edges[o - sj] + up);
+    }
+    if (j == d.je) {
+      const T lo = T(kA2) * (sx[x - 3][li] + sx[x][li]) + T(kA1) * (sx[x - 2][li] + sx[x - 1][li]);
+      qxx = T(kCc1) * (sx[x - 1][li] + sx[x][li]) + T(kCc2) * (edges[o + sj] + lo);
+    }
+    if (i == d.is + 1) {
+      const T rr = T(kA2) * (sy[r][li - 1] + sy[r][li + 2]) + T(kA1) * (sy[r][li] + sy[r][li + 1]);
+      qyy = T(kCc1) * (sy[r][li - 1] + sy[r][li]) + T(kCc2) * (edges[o - 1] + rr);
+    }
+    if (i == d.ie) {
+      const T l = T(kA2) * (sy[r][li - 3] + sy[r][li]) + T(kA1) * (sy[r][li - 2] + sy[r][li - 1]);
+      qyy = T(kCc1) * (sy[r][li - 1] + sy[r][li]) + T(kCc2) * (edges[o + 1] + l);
+    }
+    qout[o] = T(0.5) * (qxx + qyy);
+  }
+}
+
 }  // namespace
 
 template <typename T>
@@ -209,58 +313,23 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
     edges[o] = val;
   });
   FV3_HIP_CHECK_LAUNCH(who);
-  // 2: volume means
-  const Box bm{d.isc - 2, d.iec + 2, d.jsc - 2, d.jec + 2};
-  launch_box(s, d, bm, k0, nk, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
-    const int64_t o2 = at2(d, i, j, t);
-    if (i >= d.isc && i <= d.iec + 1) {
-      T q7[7], dx7[7];
-#pragma unroll
-      for (int n = 0; n < 7; ++n) {
-        q7[n] = qin[o + n - 3];
-        dx7[n] = m.dxa[o2 + n - 3];
-      }
-      qx[o] = volume_mean(q7, dx7, i, d.is, d.ie);
+  // 2 + 3: volume means and interpolation off the tile-edge lines
+  {
+    A2BArgs a;
+    a.ci0 = w ? d.isc + 1 : d.isc;
+    a.cj0 = so ? d.jsc + 1 : d.jsc;
+    a.nci = (e ? d.iec : d.iec + 1) - a.ci0 + 1;
+    a.ncj = (no ? d.jec : d.jec + 1) - a.cj0 + 1;
+    a.k0 = k0;
+    if (a.nci > 0 && a.ncj > 0) {
+      a.nbi = (a.nci + AB_TI - 1) / AB_TI;
+      const int nbj = (a.ncj + AB_TJ - 1) / AB_TJ;
+      dim3 grid((unsigned)(a.nbi * nbj), (unsigned)nk, (unsigned)d.ntiles);
+      hipLaunchKernelGGL((a2b_interp_kernel<T>), grid, dim3(AB_W, AB_NY), 0, s, d, m, a, qin,
+                         edges, qout);
+      FV3_HIP_CHECK_LAUNCH(who);
     }
-    if (j >= d.jsc && j <= d.jec + 1) {
-      T q7[7], dy7[7];
-#pragma unroll
-      for (int n = 0; n < 7; ++n) {
-        q7[n] = qin[o + (int64_t)(n - 3) * d.sj];
-        dy7[n] = m.dya[o2 + (int64_t)(n - 3) * d.sj2];
-      }
-      qy[o] = volume_mean(q7, dy7, j, d.js, d.je);
-    }
-  });
-  FV3_HIP_CHECK_LAUNCH(who);
-  // 3: interpolation off the tile-edge lines
-  const Box bi{w ? d.isc + 1 : d.isc, e ? d.iec : d.iec + 1, so ? d.jsc + 1 : d.jsc,
-               no ? d.jec : d.jec + 1};
-  launch_box(s, d, bi, k0, nk, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
-    const int64_t sj = d.sj;
-    T qxx = T(kA2) * (qx[o - 2 * sj] + qx[o + sj]) + T(kA1) * (qx[o - sj] + qx[o]);
-    T qyy = T(kA2) * (qy[o - 2] + qy[o + 1]) + T(kA1) * (qy[o - 1] + qy[o]);
-    if (j == d.js + 1) {
-      const T up = T(kA2) * (qx[o - sj] + qx[o + 2 * sj]) + T(kA1) * (qx[o] + qx[o + sj]);
-      qxx = T(kCc1) * (qx[o - sj] + qx[o]) + T(kCc2) * (edges[o - sj] + up);
-    }
-    if (j == d.je) {
-      const T lo = T(kA2) * (qx[o - 3 * sj] + qx[o]) + T(kA1) * (qx[o - 2 * sj] + qx[o - sj]);
-      qxx = T(kCc1) * (qx[o - sj] + qx[o]) + T(kCc2) * (edges[o + sj] + lo);
-    }
-    if (i == d.is + 1) {
-      const T r = T(kA2) * (qy[o - 1] + qy[o + 2]) + T(kA1) * (qy[o] + qy[o + 1]);
-      qyy = T(kCc1) * (qy[o - 1] + qy[o]) + T(kCc2) * (edges[o - 1] + r);
-    }
-    if (i == d.ie) {
-      const T l = T(kA2) * (qy[o - 3] + qy[o]) + T(kA1) * (qy[o - 2] + qy[o - 1]);
-      qyy = T(kCc1) * (qy[o - 1] + qy[o]) + T(kCc2) * (edges[o + 1] + l);
-    }
-    qout[o] = T(0.5) * (qxx + qyy);
-  });
-  FV3_HIP_CHECK_LAUNCH(who);
+  }
   if (replace && qin_w) {
     launch_box(s, d, bx, k0, nk, [=] __device__(int i, int j, int k, int t) {
       const int64_t o = at3(d, i, j, k, t);
