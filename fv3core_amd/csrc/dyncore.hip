@@ -218,33 +218,47 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------------------------------- rings
-// pk3_halo (width 2, pk3 = pe^akap) and pe_halo (width 1, pe) column sums.
+// pk3_halo (width 2, pk3 = pe^akap) and pe_halo (width 1, pe) column sums on
+// the ring of halo columns around the compute box.  The ring holds few
+// columns, so a thread owns one (ring column, level) pair and re-forms its
+// level's pe = ptop + dp[0] + ... + dp[k-1] itself, in the order of the
+// column sum, instead of one thread walking each column with a pow per level.
+// Lanes of a wave share k and own neighbouring ring columns.
 template <typename T>
 __global__ void __launch_bounds__(256)
     ring_kernel(Dom d, int width, int mode, T ptop, T akap, const T* __restrict__ delp,
                 T* __restrict__ out) {
-  const int ni = d.iec - d.isc + 1 + 2 * width, nj = d.jec - d.jsc + 1 + 2 * width;
-  const int n = (int)(blockIdx.x * 256 + threadIdx.x);
-  if (n >= ni * nj) return;
-  const int i = d.isc - width + n % ni, j = d.jsc - width + n / ni, t = (int)blockIdx.y;
-  if (i >= d.isc && i <= d.iec && j >= d.jsc && j <= d.jec) return;
-  T pe = ptop;
-  const int64_t base = at3(d, i, j, 0, t), sk = d.sk;
-  if (mode == 1) out[base] = ptop;
-  for (int k0 = 1; k0 < d.nk; k0 += KC) {
-    T dp[KC];
-#pragma unroll
-    for (int u = 0; u < KC; ++u)
-      if (k0 + u < d.nk) dp[u] = delp[base + (k0 + u - 1) * sk];
-#pragma unroll
-    for (int u = 0; u < KC; ++u) {
-      const int k = k0 + u;
-      if (k < d.nk) {
-        pe = pe + dp[u];
-        out[base + k * sk] = mode == 0 ? pow(pe, akap) : pe;
-      }
-    }
+  const int ni = d.iec - d.isc + 1, nj = d.jec - d.jsc + 1;
+  const int wi = ni + 2 * width;                   // strip length along i
+  const int nstrip = width * wi;                   // cells of the south / north strip
+  const int nside = width * nj;                    // cells of the west / east strip
+  const int nring = 2 * nstrip + 2 * nside;
+  const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (n >= (int64_t)nring * d.nk) return;
+  const int k = (int)(n / nring);
+  int c = (int)(n % nring);
+  int i, j;
+  if (c < 2 * nstrip) {
+    const int north = c >= nstrip;
+    c -= north * nstrip;
+    i = d.isc - width + c % wi;
+    j = north ? d.jec + 1 + c / wi : d.jsc - width + c / wi;
+  } else {
+    c -= 2 * nstrip;
+    const int east = c >= nside;
+    c -= east * nside;
+    i = east ? d.iec + 1 + c % width : d.isc - width + c % width;
+    j = d.jsc + c / width;
   }
+  const int t = (int)blockIdx.y;
+  const int64_t base = at3(d, i, j, 0, t), sk = d.sk;
+  if (k == 0) {
+    if (mode == 1) out[base] = ptop;
+    return;
+  }
+  T pe = ptop;
+  for (int m = 0; m < k; ++m) pe = pe + delp[base + m * sk];
+  out[base + k * sk] = mode == 0 ? pow(pe, akap) : pe;
 }
 
 // ------------------------------------------------------------------ ray_fast
@@ -287,6 +301,12 @@ __global__ void __launch_bounds__(256)
 }
 
 }  // namespace
+
+// (ring column, level) pairs of a ring of the given width around the box
+static int64_t ring_threads(const Dom& d, int width) {
+  const int ni = d.iec - d.isc + 1, nj = d.jec - d.jsc + 1;
+  return (int64_t)(2 * width * (ni + 2 * width) + 2 * width * nj) * d.nk;
+}
 
 int check_metrics_list(const fv3_metrics* g, const void* const* ptrs, int n, const char* who) {
   FV3_REQUIRE(g != nullptr, FV3_E_ARG, "%s: NULL metrics", who);
@@ -460,9 +480,9 @@ int32_t fv3_pk3_halo_f64(const fv3_domain* dom, double* pk3, const double* delp,
   FV3_REQUIRE(pk3 && delp, FV3_E_ARG, "%s: NULL pointer", who);
   if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
   Dom d = to_dom(dom);
-  const int nc = (d.iec - d.isc + 5) * (d.jec - d.jsc + 5);
-  hipLaunchKernelGGL((ring_kernel<double>), dim3((nc + 255) / 256, d.ntiles), dim3(256), 0,
-                     as_stream(stream), d, 2, 0, ptop, akap, delp, pk3);
+  const int64_t nc = ring_threads(d, 2);
+  hipLaunchKernelGGL((ring_kernel<double>), dim3((unsigned)((nc + 255) / 256), d.ntiles),
+                     dim3(256), 0, as_stream(stream), d, 2, 0, ptop, akap, delp, pk3);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
 }
@@ -476,9 +496,9 @@ int32_t fv3_pe_halo_f64(const fv3_domain* dom, double* pe, const double* delp, d
   FV3_REQUIRE(pe && delp, FV3_E_ARG, "%s: NULL pointer", who);
   if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
   Dom d = to_dom(dom);
-  const int nc = (d.iec - d.isc + 3) * (d.jec - d.jsc + 3);
-  hipLaunchKernelGGL((ring_kernel<double>), dim3((nc + 255) / 256, d.ntiles), dim3(256), 0,
-                     as_stream(stream), d, 1, 1, ptop, 0.0, delp, pe);
+  const int64_t nc = ring_threads(d, 1);
+  hipLaunchKernelGGL((ring_kernel<double>), dim3((unsigned)((nc + 255) / 256), d.ntiles),
+                     dim3(256), 0, as_stream(stream), d, 1, 1, ptop, 0.0, delp, pe);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
 }
