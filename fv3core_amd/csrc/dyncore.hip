@@ -19,28 +19,27 @@ constexpr double kDzMin = 2.0;
 // Value of the 2-cell corner-filled copy (fill_corners_2cells_x/_y,
 // corners.py:127-300) of q at (i, j): the rotated source for the 8
 // destinations of a tile's corners, q itself elsewhere.
-template <typename T>
-__device__ __forceinline__ T fill2_view(const T* q, const Dom& d, int i, int j, int64_t o,
-                                        bool ydir) {
+// The same as an offset: q at (i, j) through the fill is q[o + fill2_off].
+__device__ __forceinline__ int64_t fill2_off(const Dom& d, int i, int j, bool ydir) {
   const bool w = i == d.is - 1 || i == d.is - 2, e = i == d.ie + 1 || i == d.ie + 2;
   const bool s = j == d.js - 1 || j == d.js - 2, n = j == d.je + 1 || j == d.je + 2;
-  if (!((w || e) && (s || n))) return q[o];
+  if (!((w || e) && (s || n))) return 0;
   int di = 0, dj = 0;
   if (!ydir) {
     // destinations (is-1, js-1), (is-2, js-1) ... at rows js-1 / je+1
-    if (j != d.js - 1 && j != d.je + 1) return q[o];
+    if (j != d.js - 1 && j != d.je + 1) return 0;
     const int a = w ? d.is - 1 - i : i - d.ie - 1;  // 0 or 1 cells beyond the corner column
     const int sy = s ? 1 : -1;
     di = w ? a : -a;
     dj = sy * (1 + a);
   } else {
-    if (i != d.is - 1 && i != d.ie + 1) return q[o];
+    if (i != d.is - 1 && i != d.ie + 1) return 0;
     const int b = s ? d.js - 1 - j : j - d.je - 1;
     const int sx = w ? 1 : -1;
     di = sx * (1 + b);
     dj = s ? b : -b;
   }
-  return q[o + di + (int64_t)dj * d.sj];
+  return di + (int64_t)dj * d.sj;
 }
 
 
@@ -74,6 +73,12 @@ __device__ __forceinline__ void floor_column_up(T* __restrict__ h, int64_t sk, i
 }
 
 // ----------------------------------------------------------------- updatedzc
+// One column per lane on [isc-1, iec+1]^2, k in chunks of KZ levels: the
+// interface winds at level k come from the layer winds k-1 and k, so the
+// sweep carries the two layers above the chunk.  The corner fill of gz is
+// a per-column offset, formed once.
+constexpr int KZ = 4;
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     update_dz_c_kernel(Dom d, const T* __restrict__ dp0, const T* __restrict__ zs,
@@ -86,42 +91,75 @@ __global__ void __launch_bounds__(256)
   const int i = d.isc - 1 + n % ni, j = d.jsc - 1 + n / ni, t = (int)blockIdx.y;
   const int64_t o2 = at2(d, i, j, t);
   const int nk = d.nk;
+  const int64_t sk = d.sk, sj = d.sj;
   const T a = area[o2];
-  // interface winds at (i, j), (i+1, j) for x and (i, j), (i, j+1) for y
-  auto iface = [&](const T* vel, int64_t oo, int k) -> T {
-    const int64_t sk = d.sk;
+  const int64_t oc = at3(d, i, j, 0, t);
+  const int64_t gxm = -1 + fill2_off(d, i - 1, j, false), gx0 = fill2_off(d, i, j, false);
+  const int64_t gxp = 1 + fill2_off(d, i + 1, j, false);
+  const int64_t gym = -sj + fill2_off(d, i, j - 1, true), gy0 = fill2_off(d, i, j, true);
+  const int64_t gyp = sj + fill2_off(d, i, j + 1, true);
+  // interface value at level k from the layer values vk (k), vm1 (k-1),
+  // vm2 (k-2) and, at the top, v1 (layer 1)
+  auto iface = [&](T vk, T vm1, T vm2, T v1, int k) -> T {
     if (k == 0) {
       const T ratio = dp0[0] / (dp0[0] + dp0[1]);
-      return vel[oo] + (vel[oo] - vel[oo + sk]) * ratio;
+      return vk + (vk - v1) * ratio;
     }
     if (k == nk - 1) {
       const T ratio = dp0[k - 1] / (dp0[k - 2] + dp0[k - 1]);
-      return vel[oo - sk] + (vel[oo - sk] - vel[oo - 2 * sk]) * ratio;
+      return vm1 + (vm1 - vm2) * ratio;
     }
     const T ir = T(1) / (dp0[k - 1] + dp0[k]);
-    return (dp0[k] * vel[oo - sk] + dp0[k - 1] * vel[oo]) * ir;
+    return (dp0[k] * vm1 + dp0[k - 1] * vk) * ir;
   };
+  T u0m1 = T(0), u1m1 = T(0), v0m1 = T(0), v1m1 = T(0);  // layer k-1
+  T u0m2 = T(0), u1m2 = T(0), v0m2 = T(0), v1m2 = T(0);  // layer k-2
   T last = T(0);
-  for (int k = 0; k < nk; ++k) {
-    const int64_t o = at3(d, i, j, k, t);
-    const T xfx0 = iface(ut, o, k), xfx1 = iface(ut, o + 1, k);
-    const T yfx0 = iface(vt, o, k), yfx1 = iface(vt, o + d.sj, k);
-    const T gx_m = fill2_view(gz, d, i - 1, j, o - 1, false);
-    const T gx_0 = fill2_view(gz, d, i, j, o, false);
-    const T gx_p = fill2_view(gz, d, i + 1, j, o + 1, false);
-    const T gy_m = fill2_view(gz, d, i, j - 1, o - d.sj, true);
-    const T gy_0 = fill2_view(gz, d, i, j, o, true);
-    const T gy_p = fill2_view(gz, d, i, j + 1, o + d.sj, true);
-    const T fx0 = xfx0 * (xfx0 > T(0) ? gx_m : gx_0);
-    const T fx1 = xfx1 * (xfx1 > T(0) ? gx_0 : gx_p);
-    const T fy0 = yfx0 * (yfx0 > T(0) ? gy_m : gy_0);
-    const T fy1 = yfx1 * (yfx1 > T(0) ? gy_0 : gy_p);
-    const T v = (gy_0 * a + fx0 - fx1 + fy0 - fy1) / (a + xfx0 - xfx1 + yfx0 - yfx1);
-    gzn[o] = v;
-    last = v;
+  for (int k0 = 0; k0 < nk; k0 += KZ) {
+    T u0[KZ], u1[KZ], v0[KZ], v1[KZ], gx[KZ][3], gy[KZ][3];
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      const int64_t ol = oc + (int64_t)min(k0 + u, nk - 2) * sk;  // layer
+      const int64_t og = oc + (int64_t)min(k0 + u, nk - 1) * sk;  // interface
+      u0[u] = ut[ol];
+      u1[u] = ut[ol + 1];
+      v0[u] = vt[ol];
+      v1[u] = vt[ol + sj];
+      gx[u][0] = gz[og + gxm];
+      gx[u][1] = gz[og + gx0];
+      gx[u][2] = gz[og + gxp];
+      gy[u][0] = gz[og + gym];
+      gy[u][1] = gz[og + gy0];
+      gy[u][2] = gz[og + gyp];
+    }
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        const T xfx0 = iface(u0[u], u0m1, u0m2, u0[1], k);
+        const T xfx1 = iface(u1[u], u1m1, u1m2, u1[1], k);
+        const T yfx0 = iface(v0[u], v0m1, v0m2, v0[1], k);
+        const T yfx1 = iface(v1[u], v1m1, v1m2, v1[1], k);
+        const T fx0 = xfx0 * (xfx0 > T(0) ? gx[u][0] : gx[u][1]);
+        const T fx1 = xfx1 * (xfx1 > T(0) ? gx[u][1] : gx[u][2]);
+        const T fy0 = yfx0 * (yfx0 > T(0) ? gy[u][0] : gy[u][1]);
+        const T fy1 = yfx1 * (yfx1 > T(0) ? gy[u][1] : gy[u][2]);
+        const T v = (gy[u][1] * a + fx0 - fx1 + fy0 - fy1) / (a + xfx0 - xfx1 + yfx0 - yfx1);
+        gzn[oc + (int64_t)k * sk] = v;
+        last = v;
+        u0m2 = u0m1;
+        u1m2 = u1m1;
+        v0m2 = v0m1;
+        v1m2 = v1m1;
+        u0m1 = u0[u];
+        u1m1 = u1[u];
+        v0m1 = v0[u];
+        v1m1 = v1[u];
+      }
+    }
   }
   ws[o2] = (zs[o2] - last) * rdt;
-  floor_column_up(gzn + at3(d, i, j, 0, t), d.sk, nk, last);
+  floor_column_up(gzn + oc, sk, nk, last);
 }
 
 // ----------------------------------------------------------------- updatedzd
@@ -206,11 +244,37 @@ __global__ void __launch_bounds__(256)
   const T a = area[o2];
   const int nk = d.nk;
   const int64_t sj = d.sj;
-  for (int k = 0; k < nk; ++k) {
-    const int64_t o = at3(d, i, j, k, t);
-    const T after = (a + xfx[o] - xfx[o + 1]) + (a + yfx[o] - yfx[o + sj]) - a;
-    const T adv = (h[o] * a + fx[o] - fx[o + 1] + fy[o] - fy[o + sj]) / after;
-    h[o] = adv + (gxd[o] - gxd[o + 1] + gyd[o] - gyd[o + sj]) / a;
+  const int64_t sk = d.sk;
+  const int64_t oc = at3(d, i, j, 0, t);
+  for (int k0 = 0; k0 < nk; k0 += KZ) {
+    T xa[KZ], xb[KZ], ya[KZ], yb[KZ], hv[KZ], fa[KZ], fb[KZ], ga[KZ], gb[KZ];
+    T xg[KZ], xh[KZ], yg[KZ], yh[KZ];
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      const int64_t o = oc + (int64_t)min(k0 + u, nk - 1) * sk;
+      xa[u] = xfx[o];
+      xb[u] = xfx[o + 1];
+      ya[u] = yfx[o];
+      yb[u] = yfx[o + sj];
+      hv[u] = h[o];
+      fa[u] = fx[o];
+      fb[u] = fx[o + 1];
+      ga[u] = fy[o];
+      gb[u] = fy[o + sj];
+      xg[u] = gxd[o];
+      xh[u] = gxd[o + 1];
+      yg[u] = gyd[o];
+      yh[u] = gyd[o + sj];
+    }
+#pragma unroll
+    for (int u = 0; u < KZ; ++u) {
+      const int k = k0 + u;
+      if (k < nk) {
+        const T after = (a + xa[u] - xb[u]) + (a + ya[u] - yb[u]) - a;
+        const T adv = (hv[u] * a + fa[u] - fb[u] + ga[u] - gb[u]) / after;
+        h[oc + (int64_t)k * sk] = adv + (xg[u] - xh[u] + yg[u] - yh[u]) / a;
+      }
+    }
   }
   const T bottom = h[at3(d, i, j, nk - 1, t)];
   ws[o2] = (zs[o2] - bottom) / dt;
