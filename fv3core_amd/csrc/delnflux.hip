@@ -18,7 +18,7 @@ namespace fv3 {
 // y+4, ... so no index is ever divided.
 constexpr int DN_W = 64;            // LDS columns = TI + 6
 constexpr int DN_TI = DN_W - 6;     // interfaces per block in i
-constexpr int DN_TJ = 16;           // interfaces per block in j
+constexpr int DN_TJ = 8;            // interfaces per block in j (swept 4-32: 8 is fastest)
 constexpr int DN_H = DN_TJ + 6;     // LDS rows
 constexpr int DN_NY = 4;            // waves per block
 constexpr int DN_NT = DN_W * DN_NY;
