@@ -153,14 +153,14 @@ __device__ __forceinline__ T volume_mean(const T (&q7)[7], const T (&dx7)[7], in
 // one level.  It stages q over the block plus a 3-cell halo in LDS, forms
 // the x volume means on the rows j-3 .. j+2 and the y volume means on the
 // columns i-3 .. i+2 of its corners there, then interpolates.  Lane x of
-// every wave owns LDS column x; wave y owns rows y, y+4, ...  (The metric
+// every wave owns LDS column x; wave y owns rows y, y+8, ...  (The metric
 // terms are read from global memory: staging them in LDS as well, once per
 // block over several levels, measured slower through the lower occupancy.)
 constexpr int AB_W = 64;           // LDS columns = TI + 6
 constexpr int AB_TI = AB_W - 6;    // corners per block in i
 constexpr int AB_TJ = 16;          // corners per block in j
 constexpr int AB_H = AB_TJ + 6;    // q rows: j0-3 .. j0+TJ+2
-constexpr int AB_NY = 4;           // waves per block
+constexpr int AB_NY = 8;           // waves per block (swept 2-16 waves, 8-32 rows)
 
 struct A2BArgs {
   int ci0, cj0, nci, ncj;  // corner box (interpolation off the edge lines)
