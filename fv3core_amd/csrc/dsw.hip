@@ -858,16 +858,14 @@ int32_t fv3_d_sw_f64(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_
   const double* del6_v = static_cast<const double*>(g->del6_v);
   const double* rarea = static_cast<const double*>(g->rarea);
 
-  // transport of q: corner copies (x in place, y into the scratch), fv_tp_2d
-  // on nk + 1 levels, then del-n damping of the fluxes on nk levels
+  // transport of q: corner copies (x in place, y applied as the transport
+  // reads q), fv_tp_2d on nk + 1 levels, then del-n damping of the fluxes on
+  // nk levels
   auto transport = [&](double* q, int hord, const double* mfx_, const double* mfy_,
                        const int* nord4, const double* damp, const double* mass, double* ox,
                        double* oy) -> int {
-    int r = fv3_copy_corners_xy_f64(&d1, q, q, qy, stream);
-    if (r) return r;
-    r = (p->transport_fp32 ? fv3_fvtp2d_mixed_f64 : fv3_fvtp2d_f64)(
-        &d1, hord, q, qy, st->crx, st->cry, st->xfx, st->yfx, mfx_, mfy_, area, dxa, dya, ox, oy,
-        stream);
+    int r = fvtp2d_corner_fill_f64(&d1, hord, q, qy, st->crx, st->cry, st->xfx, st->yfx, mfx_,
+                                   mfy_, area, dxa, dya, ox, oy, p->transport_fp32 != 0, stream);
     if (r) return r;
     if (nord4) {
       r = fv3_delnflux_f64(dom, mass ? FV3_DELN_MASS : FV3_DELN_ADD, nord4, q, mass, damp,

@@ -445,12 +445,10 @@ int32_t fv3_update_dz_d_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t
   hipLaunchKernelGGL((spline_kernel<double>), dim3((ncol + 255) / 256, d.ntiles, 4), dim3(256), 0,
                      s, d, gk, beta, gamma, crx, xfx, cry, yfx, crx_i, xfx_i, cry_i, yfx_i);
   FV3_HIP_CHECK_LAUNCH(who);
-  rc = fv3_copy_corners_xy_f64(dom, height, height, qy, stream);
-  if (rc) return rc;
   const double* area = static_cast<const double*>(g->area);
-  rc = fv3_fvtp2d_f64(dom, hord_tm, height, qy, crx_i, cry_i, xfx_i, yfx_i, nullptr, nullptr,
-                      area, static_cast<const double*>(g->dxa),
-                      static_cast<const double*>(g->dya), fx, fy, stream);
+  rc = fvtp2d_corner_fill_f64(dom, hord_tm, height, qy, crx_i, cry_i, xfx_i, yfx_i, nullptr,
+                              nullptr, area, static_cast<const double*>(g->dxa),
+                              static_cast<const double*>(g->dya), fx, fy, false, stream);
   if (rc) return rc;
   rc = fv3_delnflux_f64(dom, FV3_DELN_STORE, nord4, height, nullptr, damp,
                         static_cast<const double*>(g->del6_u), static_cast<const double*>(g->del6_v),

@@ -233,11 +233,28 @@ struct ColumnPPM {
   }
 };
 
+// The y-direction corner fill applied as q_y is read.  Each present corner c
+// (0 = (is, js), 1 = (ie, js), 2 = (is, je), 3 = (ie, je)) fills the 3 x 3
+// halo cells beyond it from source (si, sj) = (ai[c] + bi[c] i + ci[c] j,
+// aj[c] + bj[c] i + cj[c] j), a rotation fitted to the reference's table on
+// the host.  Sources are never corner cells.  present = 0: q_y is read as it
+// is.  (Coefficients are picked with constant indices so that the table stays
+// in the kernel arguments.)
+struct YFill {
+  int present;  // bit c: corner c is filled
+  int ai[4], bi[4], ci[4], aj[4], bj[4], cj[4];
+};
+
+__device__ __forceinline__ int sel4(int c, int v0, int v1, int v2, int v3) {
+  return c == 0 ? v0 : c == 1 ? v1 : c == 2 ? v2 : v3;
+}
+
 // S is the storage type of every field, T the type the march computes in
 // (the same, or float for FP64 storage in the mixed-precision form).
 template <typename S, typename T, int ORD, bool EX, bool EY>
 __device__ __forceinline__ void fvtp2d_strip(
-    const Dom& d, int i0, int j0, const S* __restrict__ qx, const S* __restrict__ qy,
+    const Dom& d, const YFill& yf, int i0, int j0, const S* __restrict__ qx,
+    const S* __restrict__ qy,
     const S* __restrict__ cx, const S* __restrict__ cy, const S* __restrict__ ax,
     const S* __restrict__ ay, const S* __restrict__ ux, const S* __restrict__ uy,
     const S* __restrict__ ar, const S* __restrict__ dx, const S* __restrict__ dy,
@@ -302,6 +319,24 @@ __device__ __forceinline__ void fvtp2d_strip(
     const int f = r - 2, g = r - 3;
     v.qx = ld(qx, r, sj);
     v.qy = ld(qy, r, sj);
+    if constexpr (EX && EY) {
+      // corner strips only: the y corner fill of q_y (the row test is
+      // wave-uniform, the column test per lane)
+      const bool south = r >= d.js - 3 && r < d.js, north = r > d.je && r <= d.je + 3;
+      if (yf.present && (south || north)) {
+        const bool west = i >= d.is - 3 && i < d.is, east = i > d.ie && i <= d.ie + 3;
+        const int c = (north ? 2 : 0) + (east ? 1 : 0);
+        if ((west || east) && ((yf.present >> c) & 1)) {
+          const int si = sel4(c, yf.ai[0], yf.ai[1], yf.ai[2], yf.ai[3]) +
+                         sel4(c, yf.bi[0], yf.bi[1], yf.bi[2], yf.bi[3]) * i +
+                         sel4(c, yf.ci[0], yf.ci[1], yf.ci[2], yf.ci[3]) * r;
+          const int sjj = sel4(c, yf.aj[0], yf.aj[1], yf.aj[2], yf.aj[3]) +
+                          sel4(c, yf.bj[0], yf.bj[1], yf.bj[2], yf.bj[3]) * i +
+                          sel4(c, yf.cj[0], yf.cj[1], yf.cj[2], yf.cj[3]) * r;
+          v.qy = T(qy[(int64_t)sjj * sj + si]);
+        }
+      }
+    }
     v.cx = ld(cx, r, sj);
     v.ax = ld(ax, r, sj);
     v.ar = ld(ar, r, sj2);
@@ -363,7 +398,7 @@ __global__ void __launch_bounds__(MW)
                   const S* __restrict__ xmf, const S* __restrict__ ymf,
                   const S* __restrict__ area, const S* __restrict__ dxa,
                   const S* __restrict__ dya, S* __restrict__ fx, S* __restrict__ fy,
-                  int nbi, int nbj) {
+                  int nbi, int nbj, YFill yf) {
   const int per_plane = nbi * nbj;
   const int total = per_plane * d.nk * d.ntiles;
   const int lb = xcd_logical_block(blockIdx.x, total);
@@ -385,7 +420,8 @@ __global__ void __launch_bounds__(MW)
   const bool ex = near(i0 - 4, i0 + MTI + 4, d.is) || near(i0 - 4, i0 + MTI + 4, d.ie);
   const bool ey = near(j0 - 4, j0 + MTJ + 4, d.js) || near(j0 - 4, j0 + MTJ + 4, d.je);
 #define FV3_TP_STRIP(EX_, EY_)                                                                  \
-  fvtp2d_strip<S, T, ORD, EX_, EY_>(d, i0, j0, q_x + off3, q_y + off3, crx + off3, cry + off3,   \
+  fvtp2d_strip<S, T, ORD, EX_, EY_>(d, yf, i0, j0, q_x + off3, q_y + off3, crx + off3,        \
+                                 cry + off3,                                                  \
                                  xfx + off3, yfx + off3, ux, uy, area + off2, dxa + off2,     \
                                  dya + off2, fx + off3, fy + off3)
   if (ex && ey) {
@@ -405,7 +441,8 @@ template <typename T, typename C = T>
 static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T* q_y,
                          const T* crx, const T* cry, const T* xfx, const T* yfx,
                          const T* xmf, const T* ymf, const T* area, const T* dxa,
-                         const T* dya, T* fx, T* fy, void* stream) {
+                         const T* dya, T* fx, T* fy, void* stream,
+                         const YFill* yfill = nullptr) {
   const char* who = "fv3_fvtp2d";
   int rc = check_domain(dom, who);
   if (rc) return rc;
@@ -427,6 +464,8 @@ static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T*
   const int64_t launched = (total + kNumXcd - 1) / kNumXcd * kNumXcd;
   dim3 grid((unsigned)launched), block(MW);
   hipStream_t s = as_stream(stream);
+  YFill yf{};
+  if (yfill) yf = *yfill;
   // algorithmic bytes: q (and its y-corner copy when it is a separate
   // array), crx, cry, both area fluxes and the optional mass fluxes read
   // once, both flux fields written once, over the compute cells
@@ -437,19 +476,19 @@ static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T*
   switch (hord) {
     case 5:
       hipLaunchKernelGGL((fvtp2d_kernel<T, C, 5>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
+                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj, yf);
       break;
     case 6:
       hipLaunchKernelGGL((fvtp2d_kernel<T, C, 6>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
+                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj, yf);
       break;
     case 7:
       hipLaunchKernelGGL((fvtp2d_kernel<T, C, 7>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
+                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj, yf);
       break;
     default:
       hipLaunchKernelGGL((fvtp2d_kernel<T, C, 8>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj);
+                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj, yf);
       break;
   }
   FV3_HIP_CHECK_LAUNCH(who);
@@ -661,6 +700,136 @@ static int corners_launch(const fv3_domain* dom, const T* q, T* q_x, T* q_y, voi
     FV3_HIP_CHECK_LAUNCH(who);
   }
   return FV3_OK;
+}
+
+// fv_tp_2d of q preceded by the reference's corner copies: copy_corners_x in
+// place on q (as CopyCornersXY with q_x = q) and copy_corners_y applied as the
+// march reads q, instead of into a full copy of q.  Every y destination is
+// also an x destination and no source is a corner cell, so q after its x
+// fill, read with the y fill, is exactly the copy the two-step form builds.
+// When the two destination sets differ, the copy is built in qy_scratch.
+template <typename T, typename C>
+static int fvtp2d_corner_fill_launch(const fv3_domain* dom, int hord, T* q, T* qy_scratch,
+                                     const T* crx, const T* cry, const T* xfx, const T* yfx,
+                                     const T* xmf, const T* ymf, const T* area, const T* dxa,
+                                     const T* dya, T* fx, T* fy, void* stream) {
+  const char* who = "fv3_fvtp2d";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(q && qy_scratch, FV3_E_ARG, "%s: required pointer is NULL", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  CornerTable tab;
+  build_corner_table(dom, 3, &tab);
+  CornerTable xt;
+  xt.n = 0;
+  int ny = 0;
+  int ydi[72], ydj[72], ysi[72], ysj[72];
+  for (int e = 0; e < tab.n; ++e) {
+    if (tab.dir[e] == 0) {
+      const int m = xt.n++;
+      xt.dst_i[m] = tab.dst_i[e];
+      xt.dst_j[m] = tab.dst_j[e];
+      xt.src_i[m] = tab.src_i[e];
+      xt.src_j[m] = tab.src_j[e];
+      xt.dir[m] = 0;
+    } else {
+      ydi[ny] = tab.dst_i[e];
+      ydj[ny] = tab.dst_j[e];
+      ysi[ny] = tab.src_i[e];
+      ysj[ny] = tab.src_j[e];
+      ++ny;
+    }
+  }
+  // every y destination must be an x destination and vice versa, and each
+  // corner's 9 y entries an exact affine map of the destination
+  bool same = xt.n == ny;
+  for (int a = 0; same && a < ny; ++a) {
+    bool hit = false;
+    for (int b = 0; b < xt.n; ++b) hit = hit || (xt.dst_i[b] == ydi[a] && xt.dst_j[b] == ydj[a]);
+    same = hit;
+  }
+  YFill yf{};
+  const int ci_[4] = {dom->i_start, dom->i_end, dom->i_start, dom->i_end};
+  const int cj_[4] = {dom->j_start, dom->j_start, dom->j_end, dom->j_end};
+  int counted = 0;
+  for (int c = 0; same && c < 4; ++c) {
+    // entries of corner c: dst beyond both of its anchors
+    int idx[9], n9 = 0;
+    for (int e = 0; e < ny; ++e) {
+      const int di = ydi[e] - ci_[c], dj = ydj[e] - cj_[c];
+      const bool in_i = (c & 1) ? (di >= 1 && di <= 3) : (di >= -3 && di <= -1);
+      const bool in_j = (c & 2) ? (dj >= 1 && dj <= 3) : (dj >= -3 && dj <= -1);
+      if (in_i && in_j) {
+        if (n9 == 9) {
+          same = false;
+          break;
+        }
+        idx[n9++] = e;
+      }
+    }
+    if (!same || n9 == 0) continue;
+    if (n9 != 9) {
+      same = false;
+      break;
+    }
+    // fit s = a + b i + c j from three cells of the 3 x 3 square
+    int e0 = -1, e1 = -1, e2 = -1;
+    for (int m = 0; m < 9; ++m) {
+      const int e = idx[m];
+      if (e0 < 0) e0 = e;
+      else if (e1 < 0 && ydj[e] == ydj[e0] && ydi[e] != ydi[e0]) e1 = e;
+      else if (e2 < 0 && ydi[e] == ydi[e0] && ydj[e] != ydj[e0]) e2 = e;
+    }
+    if (e1 < 0 || e2 < 0) {
+      same = false;
+      break;
+    }
+    const int dxi = ydi[e1] - ydi[e0], dyj = ydj[e2] - ydj[e0];
+    const int bi = (ysi[e1] - ysi[e0]) / dxi, bj = (ysj[e1] - ysj[e0]) / dxi;
+    const int cI = (ysi[e2] - ysi[e0]) / dyj, cJ = (ysj[e2] - ysj[e0]) / dyj;
+    const int ai = ysi[e0] - bi * ydi[e0] - cI * ydj[e0];
+    const int aj = ysj[e0] - bj * ydi[e0] - cJ * ydj[e0];
+    for (int m = 0; m < 9; ++m) {
+      const int e = idx[m];
+      same = same && ai + bi * ydi[e] + cI * ydj[e] == ysi[e] &&
+             aj + bj * ydi[e] + cJ * ydj[e] == ysj[e];
+    }
+    yf.present |= 1 << c;
+    yf.ai[c] = ai;
+    yf.bi[c] = bi;
+    yf.ci[c] = cI;
+    yf.aj[c] = aj;
+    yf.bj[c] = bj;
+    yf.cj[c] = cJ;
+    counted += 9;
+  }
+  same = same && counted == ny;
+  if (!same) {
+    rc = corners_launch<T>(dom, q, q, qy_scratch, stream);
+    if (rc) return rc;
+    return fvtp2d_launch<T, C>(dom, hord, q, qy_scratch, crx, cry, xfx, yfx, xmf, ymf, area,
+                               dxa, dya, fx, fy, stream);
+  }
+  if (xt.n > 0) {
+    Dom d = to_dom(dom);
+    hipLaunchKernelGGL((corners_kernel<T>), dim3(1, d.nk, d.ntiles), dim3(128), 0,
+                       as_stream(stream), d, xt, q, q, q);
+    FV3_HIP_CHECK_LAUNCH(who);
+  }
+  return fvtp2d_launch<T, C>(dom, hord, q, q, crx, cry, xfx, yfx, xmf, ymf, area, dxa, dya, fx,
+                             fy, stream, &yf);
+}
+
+int fvtp2d_corner_fill_f64(const fv3_domain* dom, int hord, double* q, double* qy_scratch,
+                           const double* crx, const double* cry, const double* xfx,
+                           const double* yfx, const double* xmf, const double* ymf,
+                           const double* area, const double* dxa, const double* dya, double* fx,
+                           double* fy, bool fp32_arithmetic, void* stream) {
+  if (fp32_arithmetic)
+    return fvtp2d_corner_fill_launch<double, float>(dom, hord, q, qy_scratch, crx, cry, xfx, yfx,
+                                                    xmf, ymf, area, dxa, dya, fx, fy, stream);
+  return fvtp2d_corner_fill_launch<double, double>(dom, hord, q, qy_scratch, crx, cry, xfx, yfx,
+                                                   xmf, ymf, area, dxa, dya, fx, fy, stream);
 }
 
 }  // namespace fv3

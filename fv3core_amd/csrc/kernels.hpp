@@ -16,4 +16,14 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
 
 int check_a2b_metrics(const fv3_metrics* g, const char* who);
 
+// fv_tp_2d of q after CopyCornersXY(q, q_x = q, q_y) (corners.py:425-540),
+// with the y corner fill applied as the transport reads q instead of into a
+// copy (qy_scratch is used only when the x and y corner sets differ).  q
+// receives the x corner fill in place, as CopyCornersXY does with q_x = q.
+int fvtp2d_corner_fill_f64(const fv3_domain* dom, int hord, double* q, double* qy_scratch,
+                           const double* crx, const double* cry, const double* xfx,
+                           const double* yfx, const double* xmf, const double* ymf,
+                           const double* area, const double* dxa, const double* dya, double* fx,
+                           double* fy, bool fp32_arithmetic, void* stream);
+
 }  // namespace fv3
