@@ -64,7 +64,7 @@ def _parse():
                    choices=["dyncore", "fv_dynamics", "fvtp2d", "adjoint", "ensemble"])
     p.add_argument("--roofline-kernel", default="fvtp2d_kernel",
                    choices=["fvtp2d_kernel", "riem3_kernel", "riem_c_kernel",
-                            "map_single_kernel"])
+                            "map_single_kernel", "delnflux_kernel", "a2b_interp_kernel"])
     p.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     p.add_argument("--graph", action="store_true",
                    help="dyncore: also time the step replayed from a captured HIP graph")

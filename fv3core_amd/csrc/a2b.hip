@@ -12,6 +12,8 @@
 //             a2b_interpolation on the corners off the tile-edge lines;
 //   3 copy:   the optional copy back into qin.
 #include "kernels.hpp"
+#include "tile_config.hpp"
+#include "timing.hpp"
 
 namespace fv3 {
 
@@ -158,9 +160,9 @@ __device__ __forceinline__ T volume_mean(const T (&q7)[7], const T (&dx7)[7], in
 // block over several levels, measured slower through the lower occupancy.)
 constexpr int AB_W = 64;           // LDS columns = TI + 6
 constexpr int AB_TI = AB_W - 6;    // corners per block in i
-constexpr int AB_TJ = 16;          // corners per block in j
+constexpr int AB_TJ = FV3_AB_TJ;    // corners per block in j
 constexpr int AB_H = AB_TJ + 6;    // q rows: j0-3 .. j0+TJ+2
-constexpr int AB_NY = 8;           // waves per block (swept 2-16 waves, 8-32 rows)
+constexpr int AB_NY = FV3_AB_NY;    // waves per block
 
 struct A2BArgs {
   int ci0, cj0, nci, ncj;  // corner box (interpolation off the edge lines)
@@ -325,6 +327,9 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
       a.nbi = (a.nci + AB_TI - 1) / AB_TI;
       const int nbj = (a.ncj + AB_TJ - 1) / AB_TJ;
       dim3 grid((unsigned)(a.nbi * nbj), (unsigned)nk, (unsigned)d.ntiles);
+      // q read, qout written (per corner-level)
+      TimedLaunch timed("a2b_interp_kernel", s,
+                        2.0 * sizeof(T) * (double)a.nci * a.ncj * nk * d.ntiles);
       hipLaunchKernelGGL((a2b_interp_kernel<T>), grid, dim3(AB_W, AB_NY), 0, s, d, m, a, qin,
                          edges, qout);
       FV3_HIP_CHECK_LAUNCH(who);

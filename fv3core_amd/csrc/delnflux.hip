@@ -9,6 +9,8 @@
 // band has nord == 0 restrict the del-2 flux to the compute interfaces, as
 // the reference's regions do.
 #include "common.hpp"
+#include "tile_config.hpp"
+#include "timing.hpp"
 
 namespace fv3 {
 
@@ -18,9 +20,9 @@ namespace fv3 {
 // y+4, ... so no index is ever divided.
 constexpr int DN_W = 64;            // LDS columns = TI + 6
 constexpr int DN_TI = DN_W - 6;     // interfaces per block in i
-constexpr int DN_TJ = 8;            // interfaces per block in j (swept 4-32: 8 is fastest)
+constexpr int DN_TJ = FV3_DN_TJ;    // interfaces per block in j
 constexpr int DN_H = DN_TJ + 6;     // LDS rows
-constexpr int DN_NY = 4;            // waves per block
+constexpr int DN_NY = FV3_DN_NY;    // waves per block
 constexpr int DN_NT = DN_W * DN_NY;
 
 struct Box {
@@ -231,6 +233,10 @@ static int delnflux_launch(const fv3_domain* dom, int mode, const int32_t* nord4
   const int nbi = (nfi + DN_TI - 1) / DN_TI, nbj = (nfj + DN_TJ - 1) / DN_TJ;
   dim3 grid(nbi * nbj, d.nk, d.ntiles);
   int4 n4 = make_int4(nord4[0], nord4[1], nord4[2], nord4[3]);
+  // q read, fx / fy read and written (per cell-level of the compute domain)
+  TimedLaunch timed("delnflux_kernel", as_stream(stream),
+                    5.0 * sizeof(T) * (double)(d.iec - d.isc + 1) * (d.jec - d.jsc + 1) * d.nk *
+                        d.ntiles);
   hipLaunchKernelGGL((delnflux_kernel<T>), grid, dim3(DN_W, DN_NY), 0, as_stream(stream), d, mode, n4,
                      nmax, tab, q, mass, damp, del6_u, del6_v, rarea, fx, fy, nbi);
   FV3_HIP_CHECK_LAUNCH(who);
