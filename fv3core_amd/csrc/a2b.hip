@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(AB_W * AB_NY)
 
 template <typename T>
 int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k0, int nk,
-                    const T* qin, T* qout, T* qx, T* qy, T* edges, int replace, T* qin_w,
+                    const T* qin, T* qout, T* qx, T* edges, int replace, T* qin_w,
                     const char* who) {
   if (nk <= 0) return FV3_OK;
   Dom d = to_dom(dom);
@@ -346,8 +346,8 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
 }
 
 template int launch_a2b_ord4<double>(hipStream_t, const fv3_domain*, const Met<double>&, int, int,
-                                     const double*, double*, double*, double*, double*, int,
-                                     double*, const char*);
+                                     const double*, double*, double*, double*, int, double*,
+                                     const char*);
 
 int check_a2b_metrics(const fv3_metrics* g, const char* who) {
   FV3_REQUIRE(g && g->dxa && g->dya && g->agrid1 && g->agrid2 && g->bgrid1 && g->bgrid2 &&
@@ -374,5 +374,5 @@ extern "C" int32_t fv3_a2b_ord4_f64(const fv3_domain* dom, const fv3_metrics* g,
               FV3_E_ARG, "%s: work_slab too small", who);
   const Met<double> m = to_met<double>(g);
   return launch_a2b_ord4<double>(as_stream(stream), dom, m, 0, dom->nk, qin, qout, work,
-                                 work + work_slab, work + 2 * work_slab, replace, qin, who);
+                                 work + 2 * work_slab, replace, qin, who);
 }

@@ -677,7 +677,7 @@ void append_rotation_fill(FillTable* tab, const fv3_domain* dom, int kind, int f
 template <typename T>
 int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv3_dsw_params* p,
                  T dt, const T* wk, T* divg, T* vc, T* uc, T* delpc, T* vort, T* ke, T* a2b_qx,
-                 T* a2b_qy, T* a2b_edges, const char* who) {
+                 T* a2b_edges, const char* who) {
   const int k0 = p->div_k0;
   fv3_domain hd = *dom;
   hd.nk = dom->nk - k0;
@@ -762,8 +762,8 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
   // _vorticity_calc
   const T dddmp = (T)p->dddmp;
   if (!(dddmp < T(1e-5))) {
-    int rc = launch_a2b_ord4<T>(s, &hd, m, 0, hd.nk, wkh, vo, a2b_qx + ko, a2b_qy + ko,
-                                a2b_edges + ko, 0, nullptr, who);
+    int rc = launch_a2b_ord4<T>(s, &hd, m, 0, hd.nk, wkh, vo, a2b_qx + ko, a2b_edges + ko, 0,
+                                nullptr, who);
     if (rc) return rc;
   }
   const T adt = fabs(dt), da_min_c = (T)p->da_min_c, dd8 = (T)p->dd8;
@@ -850,7 +850,7 @@ int32_t fv3_d_sw_f64(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_
   double *ucc = W[0], *vcc = W[1], *fx = W[2], *fy = W[3], *gx = W[4], *gy = W[5];
   double *fx2 = W[6], *fy2 = W[7], *dw = W[8], *heat_s = W[9], *wk = W[10], *vort = W[11];
   double *ke = W[12], *vxd = W[13], *vyd = W[14], *ut = W[15], *vt = W[16], *qy = W[17];
-  double *a2x = W[18], *a2y = W[19], *a2e = W[20], *vort2 = W[21];
+  double *a2x = W[18], *a2e = W[20], *vort2 = W[21];  // W[19] is free
   const double* area = static_cast<const double*>(g->area);
   const double* dxa = static_cast<const double*>(g->dxa);
   const double* dya = static_cast<const double*>(g->dya);
@@ -920,7 +920,7 @@ int32_t fv3_d_sw_f64(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_
     FV3_HIP_CHECK_LAUNCH(who);
   }
   rc = run_div_high<double>(s, dom, m, p, dt, wk, st->divgd, st->vc, st->uc, st->delpc, vort, ke,
-                            a2x, a2y, a2e, who);
+                            a2x, a2e, who);
   if (rc) return rc;
   launch_vort_diff<double>(s, d, m, vort, wk, vxd, vyd, vort2, p->d_con_col);
   FV3_HIP_CHECK_LAUNCH(who);

@@ -479,15 +479,15 @@ int32_t fv3_nh_p_grad_f64(const fv3_domain* dom, const fv3_metrics* g, double* u
   hipStream_t s = as_stream(stream);
   const Met<double> m = to_met<double>(g);
   double *wk = work, *wk1 = work + work_slab, *qx = work + 2 * work_slab;
-  double *qy = work + 3 * work_slab, *edges = work + 4 * work_slab;
+  double* edges = work + 4 * work_slab;
   // a2b_k1 on pp and pk3 (levels 1..nz), a2b_kbuffer on gz (0..nz), replace
-  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pp, wk1, qx, qy, edges, 1, pp, who);
+  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pp, wk1, qx, edges, 1, pp, who);
   if (rc) return rc;
-  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pk3, wk1, qx, qy, edges, 1, pk3, who);
+  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pk3, wk1, qx, edges, 1, pk3, who);
   if (rc) return rc;
-  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz + 1, gz, wk1, qx, qy, edges, 1, gz, who);
+  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz + 1, gz, wk1, qx, edges, 1, gz, who);
   if (rc) return rc;
-  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz, delp, wk1, qx, qy, edges, 0, nullptr, who);
+  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz, delp, wk1, qx, edges, 0, nullptr, who);
   if (rc) return rc;
   Dom d = to_dom(dom);
   // set_k0_and_calc_wk on [isc, iec+1]^2
