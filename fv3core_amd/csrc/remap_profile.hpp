@@ -115,15 +115,15 @@ struct RemapArgs {
 // once instead of once per level.
 constexpr int kRemapKC = 8;
 
-// slabs: 0 edge values (km+1), 1 gam (km+1), 2 a4_2, 3 a4_3, 4 a4_4,
-//        5 copy of a4_1 (the walk of sweep 4 reads it after q1 is overwritten)
+// slabs: 0 edge values (km+1), 1 gam (km+1), 5 copy of a4_1 (the producer
+//        reads it after the walk has started overwriting q1); 2-4 unused
 //
 // Sweeps: (1) upward forward elimination of the edge-value system with the
-// running values in registers; (2) downward back substitution; (3) one
-// upward sweep that clamps the edge values and forms the interpolation
-// coefficients, with a register window over the layer means, the clamped
-// edges and the extremum flags of the neighbouring levels; (4) the
-// overlap walk, which prefetches the next source layer.  Every value is
+// running values in registers; (2) downward back substitution; (3 + 4) the
+// overlap walk, fed by a producer that clamps the edge values and forms the
+// interpolation coefficients one source layer ahead of it, with a register
+// window over the layer means, the clamped edges and the extremum flags of
+// the neighbouring levels.  Every value is
 // formed by the same operations in the same order as the stencils.
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -289,33 +289,35 @@ __global__ void __launch_bounds__(256)
     }
   }
 
-  // ---- (3) apply_constraints for level s and set_interpolation_coefficients
-  // for level s-1 at step s.  Window: a4_1 of levels s-3 .. s+2, the clamped
-  // edges of level s-1 and the flags (extm | ext5 << 1 | ext6 << 2) of levels
-  // s-2 .. s.
+  // ---- (3) + (4): apply_constraints for level s and
+  // set_interpolation_coefficients for level s-1 at producer step s, run one
+  // source layer ahead of the overlap walk that consumes them, so the limited
+  // coefficients never go through memory.  Producer window: a4_1 of levels
+  // s-3 .. s+2, the clamped edges of level s-1 and the flags (extm |
+  // ext5 << 1 | ext6 << 2) of levels s-2 .. s; a step's two loads are issued
+  // one step ahead.
   {
     T am3 = T(0), am2 = T(0), am1 = T(0);
     T a0 = q_0, ap1 = q_1;
     T qc_prev = EV[0];  // clamped edge at interface s (level 0's top edge is never clamped)
     T e2p = T(0), e3p = T(0);
     int fm2 = 0, fm1 = 0;
-    for (int s0 = 0; s0 <= km; s0 += KC) {
-      T av[KC], evv[KC];
-#pragma unroll
-      for (int u = 0; u < KC; ++u) {
-        av[u] = Q[min(s0 + u + 2, km - 1)];
-        evv[u] = EV[min(s0 + u + 1, km)];
-      }
-#pragma unroll
-      for (int u = 0; u < KC; ++u) {
-        const int s = s0 + u;
-        if (s <= km) {
-          const T ap2 = av[u];
+    int s = 0;
+    T nx_a = A1[2], nx_e = EV[1];
+    // producer step s; for s >= 1 it returns level s-1's a4_1 .. a4_4
+    struct Coef {
+      T a1, a2, a3, a4;
+    };
+    auto produce = [&]() __attribute__((always_inline)) -> Coef {
+          Coef o{T(0), T(0), T(0), T(0)};
+          const T ap2 = nx_a, wp1 = nx_e;
+          nx_a = A1[min(s + 3, km - 1)];
+          nx_e = EV[min(s + 2, km)];
           T e2 = T(0), e3 = T(0);
           int f0 = 0;
           if (s < km) {
             const T qk = qc_prev;
-            T qn = evv[u];
+            T qn = wp1;
             if (s + 1 <= km - 1) {
               const int kk = s + 1;
               const T tmp = pick(a0 > ap1, a0, ap1);
@@ -473,9 +475,7 @@ __global__ void __launch_bounds__(256)
               else
                 posdef_iv1(a1, a2, a3, a4);
             }
-            A2[k] = a2;
-            A3[k] = a3;
-            A4[k] = a4;
+            o = Coef{a1, a2, a3, a4};
           }
           am3 = am2;
           am2 = am1;
@@ -486,19 +486,27 @@ __global__ void __launch_bounds__(256)
           fm1 = f0;
           e2p = e2;
           e3p = e3;
-        }
-      }
+          ++s;
+          return o;
+    };
+    // overlap walk (map_single.py:19-66) over source layers L, with layer L's
+    // coefficients in registers and layer L+1's produced as soon as L moves
+    produce();  // step 0: no level yet
+    T c1, c2, c3, c4, n1, n2, n3, n4;
+    {
+      const Coef a = produce();  // level 0
+      c1 = a.a1;
+      c2 = a.a2;
+      c3 = a.a3;
+      c4 = a.a4;
+      const Coef b = produce();  // level 1
+      n1 = b.a1;
+      n2 = b.a2;
+      n3 = b.a3;
+      n4 = b.a4;
     }
-  }
-
-  // ---- (4) lagrangian_contributions (map_single.py:19-66): the overlap walk
-  // over source layers L, with layer L's values in registers and layer
-  // L+1's loads issued as soon as L moves
-  {
     int L = 0;
-    T pL = p_0, pR = p_1;
-    T c1 = A1[0], c2 = A2[0], c3 = A3[0], c4 = A4[0];
-    T nR = p_2, n1 = A1[1], n2 = A2[1], n3 = A3[1], n4 = A4[1];
+    T pL = p_0, pR = p_1, nR = p_2;
     auto advance = [&]() __attribute__((always_inline)) {  // L -> L + 1, for L < km - 1
       L += 1;
       pL = pR;
@@ -507,12 +515,14 @@ __global__ void __launch_bounds__(256)
       c2 = n2;
       c3 = n3;
       c4 = n4;
-      const int m = L + 1 < km ? L + 1 : km - 1;
-      nR = P1[m + 1];
-      n1 = A1[m];
-      n2 = A2[m];
-      n3 = A3[m];
-      n4 = A4[m];
+      if (L + 1 < km) {
+        nR = P1[L + 2];
+        const Coef b = produce();  // level L + 1
+        n1 = b.a1;
+        n2 = b.a2;
+        n3 = b.a3;
+        n4 = b.a4;
+      }
     };
     T p2k = P2[0];
     for (int k0 = 0; k0 < km; k0 += KC) {
