@@ -680,7 +680,7 @@ int32_t fv3_map_single_tlm_f64(const fv3_domain* dom, int32_t kord, int32_t mode
   RemapArgs a{9, mode, dom->nk, i1, j1, nib, njb, qmin, work_slab};
   Dom d = to_dom(dom);
   dim3 grid((unsigned)(((int64_t)nib * njb + 255) / 256), d.ntiles);
-  hipLaunchKernelGGL((map_single_kernel<Dual<1>>), grid, dim3(256), 0, s, d, a, dq, dp1, dp2,
+  hipLaunchKernelGGL((map_single_kernel<Dual<1>, false>), grid, dim3(256), 0, s, d, a, dq, dp1, dp2,
                      qs ? dqs : nullptr, kwork);
   hipLaunchKernelGGL(unpack_kernel, blocks(nq), dim3(256), 0, s, nq, dq, q1, dq1);
   FV3_HIP_CHECK_LAUNCH(who);

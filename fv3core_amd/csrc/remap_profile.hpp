@@ -115,17 +115,21 @@ struct RemapArgs {
 // once instead of once per level.
 constexpr int kRemapKC = 8;
 
-// slabs: 0 edge values (km+1), 1 gam (km+1), 5 copy of a4_1 (the producer
-//        reads it after the walk has started overwriting q1); 2-4 unused
+// slabs: 0 edge values (km+1), 1 gam (km+1), 2 a4_2, 3 a4_3, 4 a4_4 (not
+//        with FUSED), 5 copy of a4_1 (read after the walk has started
+//        overwriting q1)
 //
 // Sweeps: (1) upward forward elimination of the edge-value system with the
-// running values in registers; (2) downward back substitution; (3 + 4) the
-// overlap walk, fed by a producer that clamps the edge values and forms the
-// interpolation coefficients one source layer ahead of it, with a register
-// window over the layer means, the clamped edges and the extremum flags of
-// the neighbouring levels.  Every value is
+// running values in registers; (2) downward back substitution; (3) one
+// upward sweep that clamps the edge values and forms the interpolation
+// coefficients, with a register window over the layer means, the clamped
+// edges and the extremum flags of the neighbouring levels; (4) the overlap
+// walk, which prefetches the next source layer.  FUSED runs (3) as a
+// producer one source layer ahead of the walk instead, so the coefficients
+// never go through memory: fewer bytes, but a longer dependent chain per
+// column, which pays only when the launch has waves enough to cover it.  Every value is
 // formed by the same operations in the same order as the stencils.
-template <typename T>
+template <typename T, bool FUSED>
 __global__ void __launch_bounds__(256)
     map_single_kernel(Dom d, RemapArgs a, T* __restrict__ q1, const T* __restrict__ pe1,
                       const T* __restrict__ pe2, const T* __restrict__ qs,
@@ -289,6 +293,272 @@ __global__ void __launch_bounds__(256)
     }
   }
 
+  if constexpr (!FUSED) {
+  // ---- (3) apply_constraints for level s and set_interpolation_coefficients
+  // for level s-1 at step s.  Window: a4_1 of levels s-3 .. s+2, the clamped
+  // edges of level s-1 and the flags (extm | ext5 << 1 | ext6 << 2) of levels
+  // s-2 .. s.
+  {
+    T am3 = T(0), am2 = T(0), am1 = T(0);
+    T a0 = q_0, ap1 = q_1;
+    T qc_prev = EV[0];  // clamped edge at interface s (level 0's top edge is never clamped)
+    T e2p = T(0), e3p = T(0);
+    int fm2 = 0, fm1 = 0;
+    for (int s0 = 0; s0 <= km; s0 += KC) {
+      T av[KC], evv[KC];
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        av[u] = Q[min(s0 + u + 2, km - 1)];
+        evv[u] = EV[min(s0 + u + 1, km)];
+      }
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        const int s = s0 + u;
+        if (s <= km) {
+          const T ap2 = av[u];
+          T e2 = T(0), e3 = T(0);
+          int f0 = 0;
+          if (s < km) {
+            const T qk = qc_prev;
+            T qn = evv[u];
+            if (s + 1 <= km - 1) {
+              const int kk = s + 1;
+              const T tmp = pick(a0 > ap1, a0, ap1);
+              const T tmp2 = pick(a0 < ap1, a0, ap1);
+              if (kk == 1 || kk == km - 1) {
+                if (qn >= tmp) qn = tmp;
+                if (qn <= tmp2) qn = tmp2;
+              } else {
+                const T gm1 = a0 - am1, gp1 = ap2 - ap1;
+                if ((gm1 * gp1) > T(0)) {
+                  if (qn >= tmp) qn = tmp;
+                  if (qn <= tmp2) qn = tmp2;
+                } else if (gm1 > T(0)) {
+                  if (qn <= tmp2) qn = tmp2;
+                } else {
+                  if (qn >= tmp) qn = tmp;
+                  if (iv == 0) {
+                    if (qn < T(0)) qn = T(0);
+                  }
+                }
+              }
+            }
+            e2 = qk;
+            e3 = qn;
+            qc_prev = qn;
+            bool extm;
+            if (s == 0 || s == km - 1) {
+              extm = (qk - a0) * (qn - a0) > T(0);
+            } else {
+              extm = (a0 - am1) * (ap1 - a0) < T(0);
+            }
+            bool e5 = false, e6 = false;
+            if (kord > 9) {
+              const T x0 = T(2) * a0 - (qk + qn);
+              const T x1 = fabs(qk - qn);
+              const T a4 = T(3) * x0;
+              e5 = fabs(x0) > x1;
+              e6 = fabs(a4) > x1;
+            }
+            f0 = (extm ? 1 : 0) | (e5 ? 2 : 0) | (e6 ? 4 : 0);
+          }
+          if (s >= 1) {
+            const int k = s - 1;
+            const T a1 = am1;
+            T a2 = e2p, a3 = e3p, a4;
+            if (k < 2) {
+              if (iv == 0) {
+                if (k == 0 && a2 < T(0)) a2 = T(0);
+                a4 = T(3) * (T(2) * a1 - (a2 + a3));
+              } else if (iv == -1) {
+                if (k == 0 && a2 * a1 <= T(0)) a2 = T(0);
+                a4 = T(3) * (T(2) * a1 - (a2 + a3));
+              } else if (iv == 2) {
+                if (k == 0) {
+                  a2 = a1;
+                  a3 = a1;
+                  a4 = T(0);
+                } else {
+                  a4 = T(3) * (T(2) * a1 - (a2 + a3));
+                }
+              } else {
+                a4 = T(3) * (T(2) * a1 - (a2 + a3));
+              }
+              if (k == 0) {
+                if (iv != 2) posdef_iv1(a1, a2, a3, a4);
+              } else {
+                remap_constraint(a1, a2, a3, a4, (fm1 & 1) != 0);
+              }
+            } else if (k < km - 2) {
+              const T gk = am1 - am2, gp1 = a0 - am1, gp2 = ap1 - a0, gm1 = am2 - am3;
+              if (kord < 9) {
+                const T pmp_1 = a1 - gp1;
+                const T lac_1 = pmp_1 + T(1.5) * gp2;
+                T tmin = pick_min3(a1, pmp_1, lac_1);
+                T tmax0 = pick(a2 > tmin, a2, tmin);
+                T tmax = pick_max3(a1, pmp_1, lac_1);
+                a2 = pick(tmax0 < tmax, tmax0, tmax);
+                const T pmp_2 = a1 + T(2) * gp1;
+                const T lac_2 = pmp_2 - T(1.5) * gm1;
+                tmin = pick_min3(a1, pmp_2, lac_2);
+                tmax0 = pick(a3 > tmin, a3, tmin);
+                tmax = pick_max3(a1, pmp_2, lac_2);
+                a3 = pick(tmax0 < tmax, tmax0, tmax);
+                a4 = T(3) * (T(2) * a1 - (a2 + a3));
+              } else if (kord == 9) {
+                const T pmp_1 = a1 - T(2) * gp1;
+                const T lac_1 = pmp_1 + T(1.5) * gp2;
+                const T pmp_2 = a1 + T(2) * gk;
+                const T lac_2 = pmp_2 - T(1.5) * gm1;
+                const bool em = (fm1 & 1) != 0;
+                if ((em && (fm2 & 1)) || (em && (f0 & 1)) ||
+                    (em && (T(a.qmin) > T(0) && a1 < T(a.qmin)))) {
+                  a2 = a1;
+                  a3 = a1;
+                  a4 = T(0);
+                } else {
+                  a4 = T(6) * a1 - T(3) * (a2 + a3);
+                  if (fabs(a4) > fabs(a2 - a3)) {
+                    T tmin = pick_min3(a1, pmp_1, lac_1);
+                    T tmax0 = pick(a2 > tmin, a2, tmin);
+                    T tmax = pick_max3(a1, pmp_1, lac_1);
+                    a2 = pick(tmax0 < tmax, tmax0, tmax);
+                    tmin = pick_min3(a1, pmp_2, lac_2);
+                    tmax0 = pick(a3 > tmin, a3, tmin);
+                    tmax = pick_max3(a1, pmp_2, lac_2);
+                    a3 = pick(tmax0 < tmax, tmax0, tmax);
+                    a4 = T(6) * a1 - T(3) * (a2 + a3);
+                  }
+                }
+              } else {  // kord == 10
+                const T pmp_1 = a1 - T(2) * gp1;
+                const T lac_1 = pmp_1 + T(1.5) * gp2;
+                const T pmp_2 = a1 + T(2) * gk;
+                const T lac_2 = pmp_2 - T(1.5) * gm1;
+                const T tmin2 = pick_min3(a1, pmp_1, lac_1);
+                const T tmax2 = pick_max3(a1, pmp_1, lac_1);
+                const T tmp2 = pick(a2 > tmin2, a2, tmin2);
+                T tmin3 = pick(a1 < pmp_2, a1, pmp_2);
+                tmin3 = pick(lac_2 < tmin3, lac_2, tmin3);
+                T tmax3 = pick(a1 > pmp_2, a1, pmp_2);
+                tmax3 = pick(lac_2 > tmax3, lac_2, tmax3);
+                const T tmp3 = pick(a3 > tmin3, a3, tmin3);
+                const bool e5 = (fm1 & 2) != 0, e6 = (fm1 & 4) != 0;
+                const bool e5n = ((fm2 | f0) & 2) != 0;
+                const bool e6n = ((fm2 | f0) & 4) != 0;
+                if (e5) {
+                  if (e5n) {
+                    a2 = a1;
+                    a3 = a1;
+                  } else if (e6n) {
+                    a2 = pick(tmp2 < tmax2, tmp2, tmax2);
+                    a3 = pick(tmp3 < tmax3, tmp3, tmax3);
+                  }
+                } else if (e6) {
+                  if (e5n) {
+                    a2 = pick(tmp2 < tmax2, tmp2, tmax2);
+                    a3 = pick(tmp3 < tmax3, tmp3, tmax3);
+                  }
+                }
+                a4 = T(3) * (T(2) * a1 - (a2 + a3));
+              }
+              if (iv == 0) posdef_iv0(a1, a2, a3, a4);
+            } else {
+              if (k == km - 1) {
+                if (iv == 0) {
+                  if (a3 < T(0)) a3 = T(0);
+                }
+                if (iv == -1) {
+                  if (a3 * a1 <= T(0)) a3 = T(0);
+                }
+              }
+              a4 = T(3) * (T(2) * a1 - (a2 + a3));
+              if (k == km - 2)
+                remap_constraint(a1, a2, a3, a4, (fm1 & 1) != 0);
+              else
+                posdef_iv1(a1, a2, a3, a4);
+            }
+            A2[k] = a2;
+            A3[k] = a3;
+            A4[k] = a4;
+          }
+          am3 = am2;
+          am2 = am1;
+          am1 = a0;
+          a0 = ap1;
+          ap1 = ap2;
+          fm2 = fm1;
+          fm1 = f0;
+          e2p = e2;
+          e3p = e3;
+        }
+      }
+    }
+  }
+
+  // ---- (4) lagrangian_contributions (map_single.py:19-66): the overlap walk
+  // over source layers L, with layer L's values in registers and layer
+  // L+1's loads issued as soon as L moves
+  {
+    int L = 0;
+    T pL = p_0, pR = p_1;
+    T c1 = A1[0], c2 = A2[0], c3 = A3[0], c4 = A4[0];
+    T nR = p_2, n1 = A1[1], n2 = A2[1], n3 = A3[1], n4 = A4[1];
+    auto advance = [&]() __attribute__((always_inline)) {  // L -> L + 1, for L < km - 1
+      L += 1;
+      pL = pR;
+      pR = nR;
+      c1 = n1;
+      c2 = n2;
+      c3 = n3;
+      c4 = n4;
+      const int m = L + 1 < km ? L + 1 : km - 1;
+      nR = P1[m + 1];
+      n1 = A1[m];
+      n2 = A2[m];
+      n3 = A3[m];
+      n4 = A4[m];
+    };
+    T p2k = P2[0];
+    for (int k0 = 0; k0 < km; k0 += KC) {
+      T p2v[KC];
+#pragma unroll
+      for (int u = 0; u < KC; ++u) p2v[u] = P2[min(k0 + u + 1, km)];
+#pragma unroll
+      for (int u = 0; u < KC; ++u) {
+        const int k = k0 + u;
+        if (k < km) {
+          const T p2n = p2v[u];
+          const T dpl = pR - pL;
+          const T pl = (p2k - pL) / dpl;
+          T out;
+          if (p2n <= pR) {
+            const T pr = (p2n - pL) / dpl;
+            out = c2 + T(0.5) * (c4 + c3 - c2) * (pr + pl) -
+                  (c4 * T(1) / T(3)) * (pr * (pr + pl) + pl * pl);
+          } else {
+            T qsum = (pR - p2k) * (c2 + T(0.5) * (c4 + c3 - c2) * (T(1) + pl) -
+                                   (c4 * T(1) / T(3)) * (T(1) + pl * (T(1) + pl)));
+            // valid columns (pe1 bottom at or below pe2 bottom) never pass the
+            // last layer; the bound only keeps a malformed column inside its
+            // storage
+            if (L < km - 1) advance();
+            while (L < km - 1 && pR < p2n) {
+              qsum += (pR - pL) * c1;
+              advance();
+            }
+            const T dpx = p2n - pL;
+            const T esl = dpx / (pR - pL);
+            qsum += dpx * (c2 + T(0.5) * esl * (c3 - c2 + c4 * (T(1) - T(2.0 / 3.0) * esl)));
+            out = qsum / (p2n - p2k);
+          }
+          Q[k] = out;
+          p2k = p2n;
+        }
+      }
+    }
+  }
+  } else {
   // ---- (3) + (4): apply_constraints for level s and
   // set_interpolation_coefficients for level s-1 at producer step s, run one
   // source layer ahead of the overlap walk that consumes them, so the limited
@@ -562,6 +832,7 @@ __global__ void __launch_bounds__(256)
         }
       }
     }
+  }
   }
 }
 
