@@ -846,6 +846,17 @@ int32_t fv3_fvtp2d_f64(const fv3_domain* d, int32_t hord, const double* q_x, con
                                     q_y_flux, stream);
 }
 
+int32_t fv3_fvtp2d_corner_fill_f64(const fv3_domain* d, int32_t hord, double* q,
+                                   double* qy_scratch, const double* crx, const double* cry,
+                                   const double* x_area_flux, const double* y_area_flux,
+                                   const double* x_mass_flux, const double* y_mass_flux,
+                                   const double* area, const double* dxa, const double* dya,
+                                   double* q_x_flux, double* q_y_flux, void* stream) {
+  return fv3::fvtp2d_corner_fill_f64(d, hord, q, qy_scratch, crx, cry, x_area_flux, y_area_flux,
+                                     x_mass_flux, y_mass_flux, area, dxa, dya, q_x_flux,
+                                     q_y_flux, false, stream);
+}
+
 int32_t fv3_fvtp2d_f32(const fv3_domain* d, int32_t hord, const float* q_x, const float* q_y,
                        const float* crx, const float* cry, const float* x_area_flux,
                        const float* y_area_flux, const float* x_mass_flux,

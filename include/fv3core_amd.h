@@ -99,6 +99,22 @@ int32_t fv3_fvtp2d_f32(const fv3_domain* d, int32_t hord, const float* q_x,
                        const float* x_mass_flux, const float* y_mass_flux,
                        const float* area, const float* dxa, const float* dya,
                        float* q_x_flux, float* q_y_flux, void* stream);
+
+/*
+ * The corner copy with q_x = q followed by the transport of q_x = q and q_y,
+ * in one call (corners.py:425-540 then fvtp2d.py:249-360), as d_sw and
+ * updatedzd use them: q receives the x corner fill in place, and the y corner fill is
+ * applied as the transport reads q instead of into a full copy.  Returns the
+ * same fluxes, bit for bit, as the two calls.  qy_scratch (laid out like q)
+ * is written only when the domain's x and y corner sets differ, which no
+ * cube tile has.  No counterpart as one call in the reference.
+ */
+int32_t fv3_fvtp2d_corner_fill_f64(const fv3_domain* d, int32_t hord, double* q,
+                                   double* qy_scratch, const double* crx, const double* cry,
+                                   const double* x_area_flux, const double* y_area_flux,
+                                   const double* x_mass_flux, const double* y_mass_flux,
+                                   const double* area, const double* dxa, const double* dya,
+                                   double* q_x_flux, double* q_y_flux, void* stream);
 /* Mixed-precision form (north-star extension 1; no counterpart in the
  * reference, which computes in FP64 only): FP64 storages, as fv3_fvtp2d_f64,
  * with the PPM and cross-update arithmetic of the kernel in FP32 (inputs

@@ -1,0 +1,91 @@
+"""Fused forms that must return the same bits as the forms they replace.
+
+* fv3_fvtp2d_corner_fill: CopyCornersXY(q, q_x = q, q_y) + fv_tp_2d in one
+  call (the y corner fill applied as the transport reads q) against the two
+  calls, on domains with all four and with two cube corners.
+* map_single: the launcher takes the fused producer/walk form from 1024
+  waves of columns up and the four-sweep form below; columns are
+  independent, so one call on six 128 x 128 tiles (fused form) must equal
+  four calls on its quadrants (four-sweep form) bit for bit.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from fv3core_amd import _lib
+from fv3core_amd.stencils.map_single import MapSingle
+from fv3core_amd.utils import gt4py_utils as utils
+from fv3core_amd.utils.grid import to_storage
+from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
+from fv3core_amd.workloads import remap_inputs, transport_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def _like(x):
+    y = utils.empty(tuple(x.shape), batch_dims=x.dim() - 3 if x.dim() > 3 else 0)
+    y.copy_(x)
+    return y
+
+
+@pytest.mark.parametrize("hord", [5, 6, 7, 8])
+@pytest.mark.parametrize("edges", [(True, True, True, True), (True, False, True, False)])
+def test_corner_fill_transport_equals_copy_then_transport(hord, edges):
+    n, nz = 24, 8
+    d = transport_inputs(n, nz, workload=5)
+    gi = GridIndexing((n, n, nz), 3, *edges)
+    dev = {k: to_storage(v) for k, v in d.items()}
+    q_a, q_b = _like(dev["q"]), _like(dev["q"])
+    qy, scratch = torch.zeros_like(q_a), torch.zeros_like(q_a)
+    fx_a, fy_a = torch.zeros_like(q_a), torch.zeros_like(q_a)
+    fx_b, fy_b = torch.zeros_like(q_a), torch.zeros_like(q_a)
+    aux = (dev["crx"], dev["cry"], dev["x_area_flux"], dev["y_area_flux"])
+    dd = _lib.make_domain(gi, q_a, nz + 1, dev["area"],
+                          others3=(q_b, qy, scratch, fx_a, fy_a, fx_b, fy_b) + aux)
+    lib = _lib.lib()
+    s = ctypes.c_void_p(_lib.stream_handle())
+    metr = (_lib.ptr(dev["area"]), _lib.ptr(dev["dxa"]), _lib.ptr(dev["dya"]))
+    _lib.check(lib.fv3_copy_corners_xy_f64(ctypes.byref(dd), _lib.ptr(q_a), _lib.ptr(q_a),
+                                           _lib.ptr(qy), s))
+    _lib.check(lib.fv3_fvtp2d_f64(ctypes.byref(dd), hord, _lib.ptr(q_a), _lib.ptr(qy),
+                                  *[_lib.ptr(a) for a in aux], None, None, *metr,
+                                  _lib.ptr(fx_a), _lib.ptr(fy_a), s))
+    _lib.check(lib.fv3_fvtp2d_corner_fill_f64(ctypes.byref(dd), hord, _lib.ptr(q_b),
+                                              _lib.ptr(scratch), *[_lib.ptr(a) for a in aux],
+                                              None, None, *metr, _lib.ptr(fx_b),
+                                              _lib.ptr(fy_b), s))
+    torch.cuda.synchronize()
+    assert torch.equal(q_a, q_b), "the x corner fill of q differs"
+    assert torch.equal(fx_a, fx_b) and torch.equal(fy_a, fy_b), "fluxes differ"
+    assert float(fx_b.abs().max()) > 0.0
+
+
+@pytest.mark.parametrize("kord,mode", [(9, 1), (10, 0), (10, -2), (9, -1)])
+def test_map_single_fused_form_equals_four_sweep_form(kord, mode):
+    n, nz, nt = 128, 79, 6
+    gi = GridIndexing((n, n, nz), 3, True, True, True, True)
+    factory = StencilFactory(StencilConfig(), gi)
+    ins = [remap_inputs(n, nz, workload=90 + t) for t in range(nt)]
+
+    def batch(name):
+        b = utils.empty((nt,) + tuple(ins[0][name].shape), batch_dims=1)
+        for t in range(nt):
+            b[t].copy_(torch.as_tensor(ins[t][name]))
+        return b
+
+    field = "q" if mode == 0 else ("w" if mode in (-2, -1) else "pt")
+    src, pe1, pe2, qs = batch(field), batch("pe1"), batch("pe2"), batch("qs")
+    qs = qs if mode == -2 else None
+    whole = _like(src)
+    MapSingle(factory, kord, mode, gi.isc, gi.iec, gi.jsc, gi.jec)(whole, pe1, pe2, qs=qs)
+    parts = _like(src)
+    h = n // 2
+    for i1 in (gi.isc, gi.isc + h):
+        for j1 in (gi.jsc, gi.jsc + h):
+            MapSingle(factory, kord, mode, i1, i1 + h - 1, j1, j1 + h - 1)(parts, pe1, pe2,
+                                                                        qs=qs)
+    torch.cuda.synchronize()
+    assert torch.equal(whole, parts)
+    assert not torch.equal(whole, src)
