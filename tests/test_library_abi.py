@@ -54,3 +54,30 @@ def test_argument_errors_are_reported_without_a_gpu():
 def test_cpu_tensor_is_rejected():
     with pytest.raises(RuntimeError, match="GPU only"):
         _lib.require_device(torch.zeros(3))
+
+
+def test_riemann_solvers_reject_more_levels_than_their_lds_column_holds():
+    """The Riemann solvers keep one column array per wave in LDS and accept
+    4 to 128 interface levels; the check runs before any launch."""
+    lib = _lib.lib()
+    d = _lib.Domain()
+    d.isc, d.iec, d.jsc, d.jec = 3, 14, 3, 14
+    d.i_start, d.i_end, d.j_start, d.j_end = 3, 14, 3, 14
+    d.ni = d.nj = 19
+    d.nk, d.ntiles = 129, 1
+    d.sj, d.sk, d.st, d.sj2, d.st2 = 19, 19 * 19, 19 * 19 * 130, 19, 19 * 19
+    p = ctypes.c_void_p(16)
+    slab = ctypes.c_int64(19 * 19 * 130)
+    rc = lib.fv3_riem_solver_c_f64(ctypes.byref(d), ctypes.c_double(1.0), ctypes.c_double(300.0),
+                                   ctypes.c_double(0.05), *([p] * 10), slab, None)
+    assert rc == -2
+    assert "128" in lib.fv3_last_error().decode()
+    rc = lib.fv3_riem_solver3_f64(ctypes.byref(d), 1, ctypes.c_double(1.0),
+                                  ctypes.c_double(300.0), ctypes.c_double(0.05), 0,
+                                  ctypes.c_double(0.0), *([p] * 15), slab, None)
+    assert rc == -2
+    d.nk = 3
+    rc = lib.fv3_riem_solver3_f64(ctypes.byref(d), 1, ctypes.c_double(1.0),
+                                  ctypes.c_double(300.0), ctypes.c_double(0.05), 0,
+                                  ctypes.c_double(0.0), *([p] * 15), slab, None)
+    assert rc == -1
