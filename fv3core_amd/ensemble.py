@@ -1,4 +1,4 @@
-"""Ensembles of cubed-sphere states in one batch (north-star extension 2).
+"""Ensembles of cubed-sphere states in one batch.
 
 A member is a whole six-tile cube.  Members are stacked along the storages'
 tile-batch axis (member m owns tiles 6m .. 6m+5), and every kernel of the

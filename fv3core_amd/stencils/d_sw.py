@@ -172,7 +172,7 @@ class DGridShallowWaterLagrangianDynamics:
         self._params = p
         self._dev_cols = {}
         self._work = Workspace(22)
-        # mixed precision (extension 1): FP32 arithmetic in the fv_tp_2d calls
+        # mixed precision: FP32 arithmetic in the fv_tp_2d calls
         self.transport_fp32 = False
 
     def _columns(self, device):

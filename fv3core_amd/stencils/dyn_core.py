@@ -129,7 +129,7 @@ class AcousticDynamics:
         self.dgrid_shallow_water_lagrangian_dynamics = DGridShallowWaterLagrangianDynamics(
             stencil_factory, grid_data, damping_coefficients, column_namelist, nested,
             stretched_grid, dsw_cfg)
-        # mixed precision (north-star extension 1): FP32 arithmetic in d_sw's
+        # mixed precision: FP32 arithmetic in d_sw's
         # fv_tp_2d transports; storages and every other kernel stay FP64
         self.dgrid_shallow_water_lagrangian_dynamics.transport_fp32 = bool(mixed_precision)
         self.cgrid_shallow_water_lagrangian_dynamics = CGridShallowWaterDynamics(

@@ -115,7 +115,8 @@ int32_t fv3_fvtp2d_corner_fill_f64(const fv3_domain* d, int32_t hord, double* q,
                                    const double* x_mass_flux, const double* y_mass_flux,
                                    const double* area, const double* dxa, const double* dya,
                                    double* q_x_flux, double* q_y_flux, void* stream);
-/* Mixed-precision form (north-star extension 1; no counterpart in the
+
+/* Mixed-precision form (no counterpart in the
  * reference, which computes in FP64 only): FP64 storages, as fv3_fvtp2d_f64,
  * with the PPM and cross-update arithmetic of the kernel in FP32 (inputs
  * rounded to FP32 on load, fluxes widened to FP64 on store). */
@@ -377,7 +378,7 @@ typedef struct fv3_dsw_params {
   int32_t div_k0;                              /* first level with nord > 0 */
   int32_t div_nord;                            /* divergence-damping order there */
   int32_t transport_fp32;  /* nonzero: the fv_tp_2d calls use fv3_fvtp2d_mixed_f64
-                              (FP32 arithmetic on the FP64 storages; extension 1) */
+                              (FP32 arithmetic on the FP64 storages) */
   double dddmp, da_min_c;
   double dd8;                                  /* (da_min_c * d4_bg) ** (div_nord + 1) */
   double d_con;                                /* namelist d_con */

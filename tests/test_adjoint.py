@@ -1,4 +1,4 @@
-"""Tangent-linear model and adjoint of fv_tp_2d (north-star extension 3).
+"""Tangent-linear model and adjoint of fv_tp_2d.
 
 CPU: the torch restatement used as the autodiff reference (tests/torch_fvtp2d.py)
 is pinned against the numpy oracle on the periodic workload.

@@ -1,4 +1,4 @@
-"""Batched ensemble members (north-star extension 2).
+"""Batched ensemble members.
 
 A batch of several cubes must advance every member exactly as a run of that
 member alone would: every kernel treats tiles independently and the halo

@@ -1,4 +1,4 @@
-"""Mixed precision (north-star extension 1): FP32 arithmetic in the fv_tp_2d
+"""Mixed precision: FP32 arithmetic in the fv_tp_2d
 transports on FP64 storages, against the plain FP64 form.
 
 Error metric: the reference's 2|a-b|/(|a|+|b|) (test_translate.py:26-33),
