@@ -115,6 +115,214 @@ struct RemapArgs {
 // once instead of once per level.
 constexpr int kRemapKC = 8;
 
+template <typename T>
+struct RemapCoef {
+  T a1, a2, a3, a4;
+};
+
+// One step s of apply_constraints (remap_profile.py:210-280, edge at
+// interface s+1) and set_interpolation_coefficients (remap_profile.py:283-495,
+// level s-1) over a register window of the column: a4_1 of levels s-3 .. s+2,
+// the clamped edges of level s-1 and the flags (extm | ext5 << 1 | ext6 << 2)
+// of levels s-2 .. s.  Steps run for s = 0 .. km in order; both forms of
+// map_single_kernel drive the same steps, so they give the same bits.
+template <typename T>
+struct RemapLimiter {
+  int kord, iv, km;
+  double qmin;
+  T am3, am2, am1, a0, ap1;
+  T qc_prev;  // clamped edge at interface s (level 0's top edge is never clamped)
+  T e2p, e3p;
+  int fm2, fm1;
+
+  __device__ __forceinline__ RemapLimiter(int kord_, int iv_, int km_, double qmin_, T q0, T q1,
+                                          T ev0)
+      : kord(kord_), iv(iv_), km(km_), qmin(qmin_), am3(T(0)), am2(T(0)), am1(T(0)), a0(q0),
+        ap1(q1), qc_prev(ev0), e2p(T(0)), e3p(T(0)), fm2(0), fm1(0) {}
+
+  // ap2: a4_1 of level s+2 (clamped to the last level); wp1: the edge value
+  // at interface s+1 from the back substitution.  For s >= 1 stores level
+  // s-1's a4_1 .. a4_4 in o and returns true.
+  __device__ __forceinline__ bool step(int s, T ap2, T wp1, RemapCoef<T>& o) {
+    bool produced = false;
+    T e2 = T(0), e3 = T(0);
+    int f0 = 0;
+    if (s < km) {
+      const T qk = qc_prev;
+      T qn = wp1;
+      if (s + 1 <= km - 1) {
+        const int kk = s + 1;
+        const T tmp = pick(a0 > ap1, a0, ap1);
+        const T tmp2 = pick(a0 < ap1, a0, ap1);
+        if (kk == 1 || kk == km - 1) {
+          if (qn >= tmp) qn = tmp;
+          if (qn <= tmp2) qn = tmp2;
+        } else {
+          const T gm1 = a0 - am1, gp1 = ap2 - ap1;
+          if ((gm1 * gp1) > T(0)) {
+            if (qn >= tmp) qn = tmp;
+            if (qn <= tmp2) qn = tmp2;
+          } else if (gm1 > T(0)) {
+            if (qn <= tmp2) qn = tmp2;
+          } else {
+            if (qn >= tmp) qn = tmp;
+            if (iv == 0) {
+              if (qn < T(0)) qn = T(0);
+            }
+          }
+        }
+      }
+      e2 = qk;
+      e3 = qn;
+      qc_prev = qn;
+      bool extm;
+      if (s == 0 || s == km - 1) {
+        extm = (qk - a0) * (qn - a0) > T(0);
+      } else {
+        extm = (a0 - am1) * (ap1 - a0) < T(0);
+      }
+      bool e5 = false, e6 = false;
+      if (kord > 9) {
+        const T x0 = T(2) * a0 - (qk + qn);
+        const T x1 = fabs(qk - qn);
+        const T a4 = T(3) * x0;
+        e5 = fabs(x0) > x1;
+        e6 = fabs(a4) > x1;
+      }
+      f0 = (extm ? 1 : 0) | (e5 ? 2 : 0) | (e6 ? 4 : 0);
+    }
+    if (s >= 1) {
+      const int k = s - 1;
+      const T a1 = am1;
+      T a2 = e2p, a3 = e3p, a4;
+      if (k < 2) {
+        if (iv == 0) {
+          if (k == 0 && a2 < T(0)) a2 = T(0);
+          a4 = T(3) * (T(2) * a1 - (a2 + a3));
+        } else if (iv == -1) {
+          if (k == 0 && a2 * a1 <= T(0)) a2 = T(0);
+          a4 = T(3) * (T(2) * a1 - (a2 + a3));
+        } else if (iv == 2) {
+          if (k == 0) {
+            a2 = a1;
+            a3 = a1;
+            a4 = T(0);
+          } else {
+            a4 = T(3) * (T(2) * a1 - (a2 + a3));
+          }
+        } else {
+          a4 = T(3) * (T(2) * a1 - (a2 + a3));
+        }
+        if (k == 0) {
+          if (iv != 2) posdef_iv1(a1, a2, a3, a4);
+        } else {
+          remap_constraint(a1, a2, a3, a4, (fm1 & 1) != 0);
+        }
+      } else if (k < km - 2) {
+        const T gk = am1 - am2, gp1 = a0 - am1, gp2 = ap1 - a0, gm1 = am2 - am3;
+        if (kord < 9) {
+          const T pmp_1 = a1 - gp1;
+          const T lac_1 = pmp_1 + T(1.5) * gp2;
+          T tmin = pick_min3(a1, pmp_1, lac_1);
+          T tmax0 = pick(a2 > tmin, a2, tmin);
+          T tmax = pick_max3(a1, pmp_1, lac_1);
+          a2 = pick(tmax0 < tmax, tmax0, tmax);
+          const T pmp_2 = a1 + T(2) * gp1;
+          const T lac_2 = pmp_2 - T(1.5) * gm1;
+          tmin = pick_min3(a1, pmp_2, lac_2);
+          tmax0 = pick(a3 > tmin, a3, tmin);
+          tmax = pick_max3(a1, pmp_2, lac_2);
+          a3 = pick(tmax0 < tmax, tmax0, tmax);
+          a4 = T(3) * (T(2) * a1 - (a2 + a3));
+        } else if (kord == 9) {
+          const T pmp_1 = a1 - T(2) * gp1;
+          const T lac_1 = pmp_1 + T(1.5) * gp2;
+          const T pmp_2 = a1 + T(2) * gk;
+          const T lac_2 = pmp_2 - T(1.5) * gm1;
+          const bool em = (fm1 & 1) != 0;
+          if ((em && (fm2 & 1)) || (em && (f0 & 1)) ||
+              (em && (T(qmin) > T(0) && a1 < T(qmin)))) {
+            a2 = a1;
+            a3 = a1;
+            a4 = T(0);
+          } else {
+            a4 = T(6) * a1 - T(3) * (a2 + a3);
+            if (fabs(a4) > fabs(a2 - a3)) {
+              T tmin = pick_min3(a1, pmp_1, lac_1);
+              T tmax0 = pick(a2 > tmin, a2, tmin);
+              T tmax = pick_max3(a1, pmp_1, lac_1);
+              a2 = pick(tmax0 < tmax, tmax0, tmax);
+              tmin = pick_min3(a1, pmp_2, lac_2);
+              tmax0 = pick(a3 > tmin, a3, tmin);
+              tmax = pick_max3(a1, pmp_2, lac_2);
+              a3 = pick(tmax0 < tmax, tmax0, tmax);
+              a4 = T(6) * a1 - T(3) * (a2 + a3);
+            }
+          }
+        } else {  // kord == 10
+          const T pmp_1 = a1 - T(2) * gp1;
+          const T lac_1 = pmp_1 + T(1.5) * gp2;
+          const T pmp_2 = a1 + T(2) * gk;
+          const T lac_2 = pmp_2 - T(1.5) * gm1;
+          const T tmin2 = pick_min3(a1, pmp_1, lac_1);
+          const T tmax2 = pick_max3(a1, pmp_1, lac_1);
+          const T tmp2 = pick(a2 > tmin2, a2, tmin2);
+          T tmin3 = pick(a1 < pmp_2, a1, pmp_2);
+          tmin3 = pick(lac_2 < tmin3, lac_2, tmin3);
+          T tmax3 = pick(a1 > pmp_2, a1, pmp_2);
+          tmax3 = pick(lac_2 > tmax3, lac_2, tmax3);
+          const T tmp3 = pick(a3 > tmin3, a3, tmin3);
+          const bool e5 = (fm1 & 2) != 0, e6 = (fm1 & 4) != 0;
+          const bool e5n = ((fm2 | f0) & 2) != 0;
+          const bool e6n = ((fm2 | f0) & 4) != 0;
+          if (e5) {
+            if (e5n) {
+              a2 = a1;
+              a3 = a1;
+            } else if (e6n) {
+              a2 = pick(tmp2 < tmax2, tmp2, tmax2);
+              a3 = pick(tmp3 < tmax3, tmp3, tmax3);
+            }
+          } else if (e6) {
+            if (e5n) {
+              a2 = pick(tmp2 < tmax2, tmp2, tmax2);
+              a3 = pick(tmp3 < tmax3, tmp3, tmax3);
+            }
+          }
+          a4 = T(3) * (T(2) * a1 - (a2 + a3));
+        }
+        if (iv == 0) posdef_iv0(a1, a2, a3, a4);
+      } else {
+        if (k == km - 1) {
+          if (iv == 0) {
+            if (a3 < T(0)) a3 = T(0);
+          }
+          if (iv == -1) {
+            if (a3 * a1 <= T(0)) a3 = T(0);
+          }
+        }
+        a4 = T(3) * (T(2) * a1 - (a2 + a3));
+        if (k == km - 2)
+          remap_constraint(a1, a2, a3, a4, (fm1 & 1) != 0);
+        else
+          posdef_iv1(a1, a2, a3, a4);
+      }
+      o = RemapCoef<T>{a1, a2, a3, a4};
+      produced = true;
+    }
+    am3 = am2;
+    am2 = am1;
+    am1 = a0;
+    a0 = ap1;
+    ap1 = ap2;
+    fm2 = fm1;
+    fm1 = f0;
+    e2p = e2;
+    e3p = e3;
+    return produced;
+  }
+};
+
 // slabs: 0 edge values (km+1), 1 gam (km+1), 2 a4_2, 3 a4_3, 4 a4_4 (not
 //        with FUSED), 5 copy of a4_1 (read after the walk has started
 //        overwriting q1)
@@ -294,16 +502,10 @@ __global__ void __launch_bounds__(256)
   }
 
   if constexpr (!FUSED) {
-  // ---- (3) apply_constraints for level s and set_interpolation_coefficients
-  // for level s-1 at step s.  Window: a4_1 of levels s-3 .. s+2, the clamped
-  // edges of level s-1 and the flags (extm | ext5 << 1 | ext6 << 2) of levels
-  // s-2 .. s.
+  // ---- (3) one upward sweep of limiter steps; step s writes level s-1's
+  // a4_2 .. a4_4
   {
-    T am3 = T(0), am2 = T(0), am1 = T(0);
-    T a0 = q_0, ap1 = q_1;
-    T qc_prev = EV[0];  // clamped edge at interface s (level 0's top edge is never clamped)
-    T e2p = T(0), e3p = T(0);
-    int fm2 = 0, fm1 = 0;
+    RemapLimiter<T> lim(kord, iv, km, a.qmin, q_0, q_1, EV[0]);
     for (int s0 = 0; s0 <= km; s0 += KC) {
       T av[KC], evv[KC];
 #pragma unroll
@@ -314,183 +516,11 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int u = 0; u < KC; ++u) {
         const int s = s0 + u;
-        if (s <= km) {
-          const T ap2 = av[u];
-          T e2 = T(0), e3 = T(0);
-          int f0 = 0;
-          if (s < km) {
-            const T qk = qc_prev;
-            T qn = evv[u];
-            if (s + 1 <= km - 1) {
-              const int kk = s + 1;
-              const T tmp = pick(a0 > ap1, a0, ap1);
-              const T tmp2 = pick(a0 < ap1, a0, ap1);
-              if (kk == 1 || kk == km - 1) {
-                if (qn >= tmp) qn = tmp;
-                if (qn <= tmp2) qn = tmp2;
-              } else {
-                const T gm1 = a0 - am1, gp1 = ap2 - ap1;
-                if ((gm1 * gp1) > T(0)) {
-                  if (qn >= tmp) qn = tmp;
-                  if (qn <= tmp2) qn = tmp2;
-                } else if (gm1 > T(0)) {
-                  if (qn <= tmp2) qn = tmp2;
-                } else {
-                  if (qn >= tmp) qn = tmp;
-                  if (iv == 0) {
-                    if (qn < T(0)) qn = T(0);
-                  }
-                }
-              }
-            }
-            e2 = qk;
-            e3 = qn;
-            qc_prev = qn;
-            bool extm;
-            if (s == 0 || s == km - 1) {
-              extm = (qk - a0) * (qn - a0) > T(0);
-            } else {
-              extm = (a0 - am1) * (ap1 - a0) < T(0);
-            }
-            bool e5 = false, e6 = false;
-            if (kord > 9) {
-              const T x0 = T(2) * a0 - (qk + qn);
-              const T x1 = fabs(qk - qn);
-              const T a4 = T(3) * x0;
-              e5 = fabs(x0) > x1;
-              e6 = fabs(a4) > x1;
-            }
-            f0 = (extm ? 1 : 0) | (e5 ? 2 : 0) | (e6 ? 4 : 0);
-          }
-          if (s >= 1) {
-            const int k = s - 1;
-            const T a1 = am1;
-            T a2 = e2p, a3 = e3p, a4;
-            if (k < 2) {
-              if (iv == 0) {
-                if (k == 0 && a2 < T(0)) a2 = T(0);
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              } else if (iv == -1) {
-                if (k == 0 && a2 * a1 <= T(0)) a2 = T(0);
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              } else if (iv == 2) {
-                if (k == 0) {
-                  a2 = a1;
-                  a3 = a1;
-                  a4 = T(0);
-                } else {
-                  a4 = T(3) * (T(2) * a1 - (a2 + a3));
-                }
-              } else {
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              }
-              if (k == 0) {
-                if (iv != 2) posdef_iv1(a1, a2, a3, a4);
-              } else {
-                remap_constraint(a1, a2, a3, a4, (fm1 & 1) != 0);
-              }
-            } else if (k < km - 2) {
-              const T gk = am1 - am2, gp1 = a0 - am1, gp2 = ap1 - a0, gm1 = am2 - am3;
-              if (kord < 9) {
-                const T pmp_1 = a1 - gp1;
-                const T lac_1 = pmp_1 + T(1.5) * gp2;
-                T tmin = pick_min3(a1, pmp_1, lac_1);
-                T tmax0 = pick(a2 > tmin, a2, tmin);
-                T tmax = pick_max3(a1, pmp_1, lac_1);
-                a2 = pick(tmax0 < tmax, tmax0, tmax);
-                const T pmp_2 = a1 + T(2) * gp1;
-                const T lac_2 = pmp_2 - T(1.5) * gm1;
-                tmin = pick_min3(a1, pmp_2, lac_2);
-                tmax0 = pick(a3 > tmin, a3, tmin);
-                tmax = pick_max3(a1, pmp_2, lac_2);
-                a3 = pick(tmax0 < tmax, tmax0, tmax);
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              } else if (kord == 9) {
-                const T pmp_1 = a1 - T(2) * gp1;
-                const T lac_1 = pmp_1 + T(1.5) * gp2;
-                const T pmp_2 = a1 + T(2) * gk;
-                const T lac_2 = pmp_2 - T(1.5) * gm1;
-                const bool em = (fm1 & 1) != 0;
-                if ((em && (fm2 & 1)) || (em && (f0 & 1)) ||
-                    (em && (T(a.qmin) > T(0) && a1 < T(a.qmin)))) {
-                  a2 = a1;
-                  a3 = a1;
-                  a4 = T(0);
-                } else {
-                  a4 = T(6) * a1 - T(3) * (a2 + a3);
-                  if (fabs(a4) > fabs(a2 - a3)) {
-                    T tmin = pick_min3(a1, pmp_1, lac_1);
-                    T tmax0 = pick(a2 > tmin, a2, tmin);
-                    T tmax = pick_max3(a1, pmp_1, lac_1);
-                    a2 = pick(tmax0 < tmax, tmax0, tmax);
-                    tmin = pick_min3(a1, pmp_2, lac_2);
-                    tmax0 = pick(a3 > tmin, a3, tmin);
-                    tmax = pick_max3(a1, pmp_2, lac_2);
-                    a3 = pick(tmax0 < tmax, tmax0, tmax);
-                    a4 = T(6) * a1 - T(3) * (a2 + a3);
-                  }
-                }
-              } else {  // kord == 10
-                const T pmp_1 = a1 - T(2) * gp1;
-                const T lac_1 = pmp_1 + T(1.5) * gp2;
-                const T pmp_2 = a1 + T(2) * gk;
-                const T lac_2 = pmp_2 - T(1.5) * gm1;
-                const T tmin2 = pick_min3(a1, pmp_1, lac_1);
-                const T tmax2 = pick_max3(a1, pmp_1, lac_1);
-                const T tmp2 = pick(a2 > tmin2, a2, tmin2);
-                T tmin3 = pick(a1 < pmp_2, a1, pmp_2);
-                tmin3 = pick(lac_2 < tmin3, lac_2, tmin3);
-                T tmax3 = pick(a1 > pmp_2, a1, pmp_2);
-                tmax3 = pick(lac_2 > tmax3, lac_2, tmax3);
-                const T tmp3 = pick(a3 > tmin3, a3, tmin3);
-                const bool e5 = (fm1 & 2) != 0, e6 = (fm1 & 4) != 0;
-                const bool e5n = ((fm2 | f0) & 2) != 0;
-                const bool e6n = ((fm2 | f0) & 4) != 0;
-                if (e5) {
-                  if (e5n) {
-                    a2 = a1;
-                    a3 = a1;
-                  } else if (e6n) {
-                    a2 = pick(tmp2 < tmax2, tmp2, tmax2);
-                    a3 = pick(tmp3 < tmax3, tmp3, tmax3);
-                  }
-                } else if (e6) {
-                  if (e5n) {
-                    a2 = pick(tmp2 < tmax2, tmp2, tmax2);
-                    a3 = pick(tmp3 < tmax3, tmp3, tmax3);
-                  }
-                }
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              }
-              if (iv == 0) posdef_iv0(a1, a2, a3, a4);
-            } else {
-              if (k == km - 1) {
-                if (iv == 0) {
-                  if (a3 < T(0)) a3 = T(0);
-                }
-                if (iv == -1) {
-                  if (a3 * a1 <= T(0)) a3 = T(0);
-                }
-              }
-              a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              if (k == km - 2)
-                remap_constraint(a1, a2, a3, a4, (fm1 & 1) != 0);
-              else
-                posdef_iv1(a1, a2, a3, a4);
-            }
-            A2[k] = a2;
-            A3[k] = a3;
-            A4[k] = a4;
-          }
-          am3 = am2;
-          am2 = am1;
-          am1 = a0;
-          a0 = ap1;
-          ap1 = ap2;
-          fm2 = fm1;
-          fm1 = f0;
-          e2p = e2;
-          e3p = e3;
+        RemapCoef<T> o;
+        if (s <= km && lim.step(s, av[u], evv[u], o)) {
+          A2[s - 1] = o.a2;
+          A3[s - 1] = o.a3;
+          A4[s - 1] = o.a4;
         }
       }
     }
@@ -559,217 +589,34 @@ __global__ void __launch_bounds__(256)
     }
   }
   } else {
-  // ---- (3) + (4): apply_constraints for level s and
-  // set_interpolation_coefficients for level s-1 at producer step s, run one
-  // source layer ahead of the overlap walk that consumes them, so the limited
-  // coefficients never go through memory.  Producer window: a4_1 of levels
-  // s-3 .. s+2, the clamped edges of level s-1 and the flags (extm |
-  // ext5 << 1 | ext6 << 2) of levels s-2 .. s; a step's two loads are issued
-  // one step ahead.
+  // ---- (3) + (4): the limiter steps run as a producer one source layer
+  // ahead of the overlap walk that consumes them, so the limited coefficients
+  // never go through memory; a step's two loads are issued one step ahead.
   {
-    T am3 = T(0), am2 = T(0), am1 = T(0);
-    T a0 = q_0, ap1 = q_1;
-    T qc_prev = EV[0];  // clamped edge at interface s (level 0's top edge is never clamped)
-    T e2p = T(0), e3p = T(0);
-    int fm2 = 0, fm1 = 0;
+    RemapLimiter<T> lim(kord, iv, km, a.qmin, q_0, q_1, EV[0]);
     int s = 0;
     T nx_a = A1[2], nx_e = EV[1];
     // producer step s; for s >= 1 it returns level s-1's a4_1 .. a4_4
-    struct Coef {
-      T a1, a2, a3, a4;
-    };
-    auto produce = [&]() __attribute__((always_inline)) -> Coef {
-          Coef o{T(0), T(0), T(0), T(0)};
-          const T ap2 = nx_a, wp1 = nx_e;
-          nx_a = A1[min(s + 3, km - 1)];
-          nx_e = EV[min(s + 2, km)];
-          T e2 = T(0), e3 = T(0);
-          int f0 = 0;
-          if (s < km) {
-            const T qk = qc_prev;
-            T qn = wp1;
-            if (s + 1 <= km - 1) {
-              const int kk = s + 1;
-              const T tmp = pick(a0 > ap1, a0, ap1);
-              const T tmp2 = pick(a0 < ap1, a0, ap1);
-              if (kk == 1 || kk == km - 1) {
-                if (qn >= tmp) qn = tmp;
-                if (qn <= tmp2) qn = tmp2;
-              } else {
-                const T gm1 = a0 - am1, gp1 = ap2 - ap1;
-                if ((gm1 * gp1) > T(0)) {
-                  if (qn >= tmp) qn = tmp;
-                  if (qn <= tmp2) qn = tmp2;
-                } else if (gm1 > T(0)) {
-                  if (qn <= tmp2) qn = tmp2;
-                } else {
-                  if (qn >= tmp) qn = tmp;
-                  if (iv == 0) {
-                    if (qn < T(0)) qn = T(0);
-                  }
-                }
-              }
-            }
-            e2 = qk;
-            e3 = qn;
-            qc_prev = qn;
-            bool extm;
-            if (s == 0 || s == km - 1) {
-              extm = (qk - a0) * (qn - a0) > T(0);
-            } else {
-              extm = (a0 - am1) * (ap1 - a0) < T(0);
-            }
-            bool e5 = false, e6 = false;
-            if (kord > 9) {
-              const T x0 = T(2) * a0 - (qk + qn);
-              const T x1 = fabs(qk - qn);
-              const T a4 = T(3) * x0;
-              e5 = fabs(x0) > x1;
-              e6 = fabs(a4) > x1;
-            }
-            f0 = (extm ? 1 : 0) | (e5 ? 2 : 0) | (e6 ? 4 : 0);
-          }
-          if (s >= 1) {
-            const int k = s - 1;
-            const T a1 = am1;
-            T a2 = e2p, a3 = e3p, a4;
-            if (k < 2) {
-              if (iv == 0) {
-                if (k == 0 && a2 < T(0)) a2 = T(0);
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              } else if (iv == -1) {
-                if (k == 0 && a2 * a1 <= T(0)) a2 = T(0);
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              } else if (iv == 2) {
-                if (k == 0) {
-                  a2 = a1;
-                  a3 = a1;
-                  a4 = T(0);
-                } else {
-                  a4 = T(3) * (T(2) * a1 - (a2 + a3));
-                }
-              } else {
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              }
-              if (k == 0) {
-                if (iv != 2) posdef_iv1(a1, a2, a3, a4);
-              } else {
-                remap_constraint(a1, a2, a3, a4, (fm1 & 1) != 0);
-              }
-            } else if (k < km - 2) {
-              const T gk = am1 - am2, gp1 = a0 - am1, gp2 = ap1 - a0, gm1 = am2 - am3;
-              if (kord < 9) {
-                const T pmp_1 = a1 - gp1;
-                const T lac_1 = pmp_1 + T(1.5) * gp2;
-                T tmin = pick_min3(a1, pmp_1, lac_1);
-                T tmax0 = pick(a2 > tmin, a2, tmin);
-                T tmax = pick_max3(a1, pmp_1, lac_1);
-                a2 = pick(tmax0 < tmax, tmax0, tmax);
-                const T pmp_2 = a1 + T(2) * gp1;
-                const T lac_2 = pmp_2 - T(1.5) * gm1;
-                tmin = pick_min3(a1, pmp_2, lac_2);
-                tmax0 = pick(a3 > tmin, a3, tmin);
-                tmax = pick_max3(a1, pmp_2, lac_2);
-                a3 = pick(tmax0 < tmax, tmax0, tmax);
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              } else if (kord == 9) {
-                const T pmp_1 = a1 - T(2) * gp1;
-                const T lac_1 = pmp_1 + T(1.5) * gp2;
-                const T pmp_2 = a1 + T(2) * gk;
-                const T lac_2 = pmp_2 - T(1.5) * gm1;
-                const bool em = (fm1 & 1) != 0;
-                if ((em && (fm2 & 1)) || (em && (f0 & 1)) ||
-                    (em && (T(a.qmin) > T(0) && a1 < T(a.qmin)))) {
-                  a2 = a1;
-                  a3 = a1;
-                  a4 = T(0);
-                } else {
-                  a4 = T(6) * a1 - T(3) * (a2 + a3);
-                  if (fabs(a4) > fabs(a2 - a3)) {
-                    T tmin = pick_min3(a1, pmp_1, lac_1);
-                    T tmax0 = pick(a2 > tmin, a2, tmin);
-                    T tmax = pick_max3(a1, pmp_1, lac_1);
-                    a2 = pick(tmax0 < tmax, tmax0, tmax);
-                    tmin = pick_min3(a1, pmp_2, lac_2);
-                    tmax0 = pick(a3 > tmin, a3, tmin);
-                    tmax = pick_max3(a1, pmp_2, lac_2);
-                    a3 = pick(tmax0 < tmax, tmax0, tmax);
-                    a4 = T(6) * a1 - T(3) * (a2 + a3);
-                  }
-                }
-              } else {  // kord == 10
-                const T pmp_1 = a1 - T(2) * gp1;
-                const T lac_1 = pmp_1 + T(1.5) * gp2;
-                const T pmp_2 = a1 + T(2) * gk;
-                const T lac_2 = pmp_2 - T(1.5) * gm1;
-                const T tmin2 = pick_min3(a1, pmp_1, lac_1);
-                const T tmax2 = pick_max3(a1, pmp_1, lac_1);
-                const T tmp2 = pick(a2 > tmin2, a2, tmin2);
-                T tmin3 = pick(a1 < pmp_2, a1, pmp_2);
-                tmin3 = pick(lac_2 < tmin3, lac_2, tmin3);
-                T tmax3 = pick(a1 > pmp_2, a1, pmp_2);
-                tmax3 = pick(lac_2 > tmax3, lac_2, tmax3);
-                const T tmp3 = pick(a3 > tmin3, a3, tmin3);
-                const bool e5 = (fm1 & 2) != 0, e6 = (fm1 & 4) != 0;
-                const bool e5n = ((fm2 | f0) & 2) != 0;
-                const bool e6n = ((fm2 | f0) & 4) != 0;
-                if (e5) {
-                  if (e5n) {
-                    a2 = a1;
-                    a3 = a1;
-                  } else if (e6n) {
-                    a2 = pick(tmp2 < tmax2, tmp2, tmax2);
-                    a3 = pick(tmp3 < tmax3, tmp3, tmax3);
-                  }
-                } else if (e6) {
-                  if (e5n) {
-                    a2 = pick(tmp2 < tmax2, tmp2, tmax2);
-                    a3 = pick(tmp3 < tmax3, tmp3, tmax3);
-                  }
-                }
-                a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              }
-              if (iv == 0) posdef_iv0(a1, a2, a3, a4);
-            } else {
-              if (k == km - 1) {
-                if (iv == 0) {
-                  if (a3 < T(0)) a3 = T(0);
-                }
-                if (iv == -1) {
-                  if (a3 * a1 <= T(0)) a3 = T(0);
-                }
-              }
-              a4 = T(3) * (T(2) * a1 - (a2 + a3));
-              if (k == km - 2)
-                remap_constraint(a1, a2, a3, a4, (fm1 & 1) != 0);
-              else
-                posdef_iv1(a1, a2, a3, a4);
-            }
-            o = Coef{a1, a2, a3, a4};
-          }
-          am3 = am2;
-          am2 = am1;
-          am1 = a0;
-          a0 = ap1;
-          ap1 = ap2;
-          fm2 = fm1;
-          fm1 = f0;
-          e2p = e2;
-          e3p = e3;
-          ++s;
-          return o;
+    auto produce = [&]() __attribute__((always_inline)) -> RemapCoef<T> {
+      RemapCoef<T> o{T(0), T(0), T(0), T(0)};
+      const T ap2 = nx_a, wp1 = nx_e;
+      nx_a = A1[min(s + 3, km - 1)];
+      nx_e = EV[min(s + 2, km)];
+      lim.step(s, ap2, wp1, o);
+      ++s;
+      return o;
     };
     // overlap walk (map_single.py:19-66) over source layers L, with layer L's
     // coefficients in registers and layer L+1's produced as soon as L moves
     produce();  // step 0: no level yet
     T c1, c2, c3, c4, n1, n2, n3, n4;
     {
-      const Coef a = produce();  // level 0
+      const RemapCoef<T> a = produce();  // level 0
       c1 = a.a1;
       c2 = a.a2;
       c3 = a.a3;
       c4 = a.a4;
-      const Coef b = produce();  // level 1
+      const RemapCoef<T> b = produce();  // level 1
       n1 = b.a1;
       n2 = b.a2;
       n3 = b.a3;
@@ -787,7 +634,7 @@ __global__ void __launch_bounds__(256)
       c4 = n4;
       if (L + 1 < km) {
         nR = P1[L + 2];
-        const Coef b = produce();  // level L + 1
+        const RemapCoef<T> b = produce();  // level L + 1
         n1 = b.a1;
         n2 = b.a2;
         n3 = b.a3;
