@@ -20,7 +20,11 @@ static int map_single_launch(const fv3_domain* dom, int kord, int iv, int i1, in
   FV3_REQUIRE(q1 && pe1 && pe2 && work, FV3_E_ARG, "%s: NULL pointer", who);
   FV3_REQUIRE(kord <= 10, FV3_E_UNSUPPORTED, "%s: kord %d not implemented", who, kord);
   const int akord = kord < 0 ? -kord : kord;
-  FV3_REQUIRE(akord >= 0, FV3_E_UNSUPPORTED, "%s: bad kord", who);
+  // The reference also accepts kord <= -11: for 10 < |kord| <= 16 it runs no
+  // inner-level limiter and above 16 no constraints at all.  Those forms are
+  // not built; they are rejected instead of running the kord-10 limiter.
+  FV3_REQUIRE(akord <= 10, FV3_E_UNSUPPORTED,
+              "%s: |kord| %d > 10 not implemented (only kord in -10..10)", who, akord);
   const int km = dom->nk;
   FV3_REQUIRE(km >= 6, FV3_E_ARG, "%s: need at least 6 layers (got %d)", who, km);
   FV3_REQUIRE(i1 >= 0 && j1 >= 0 && i2 < dom->ni && j2 < dom->nj && i2 >= i1 - 1 &&

@@ -70,8 +70,8 @@ class AcousticDynamics:
                  mixed_precision: bool = False):
         gi = stencil_factory.grid_indexing
         self._use_graph = bool(use_graph)
-        self._graph = None
-        self._graph_key = None
+        self._graph = None  # the graph recorded last
+        self._graphs = {}
         self._seen_keys = set()
         if config.d_ext != 0:
             raise NotImplementedError("d_ext != 0 is not implemented")
@@ -192,19 +192,23 @@ class AcousticDynamics:
             return self._step(state)
         state.__dict__.update(self._temporaries)
         key = self._key(state)
-        if self._graph is not None and key == self._graph_key:
-            self._graph.replay()
+        graph = self._graphs.get(key)
+        if graph is not None:
+            graph.replay()
             return None
         if key not in self._seen_keys:
             # first call for these storages: eager, fills every cache
             self._seen_keys.add(key)
             return self._step(state)
+        # one graph per (storages, n_map, dt ...) combination: with k_split > 1
+        # the calls of a step alternate n_map and each keeps its own graph
         graph = torch.cuda.CUDAGraph()
         torch.cuda.synchronize()
         with torch.cuda.graph(graph):
             self._step(state)
         torch.cuda.synchronize()
-        self._graph, self._graph_key = graph, key
+        self._graphs[key] = graph
+        self._graph = graph
         graph.replay()
         return None
 

@@ -104,8 +104,7 @@ class DynamicalCore:
             stencil_factory=stencil_factory, grid_data=grid_data,
             damping_coefficients=damping_coefficients, grid_type=config.grid_type,
             hord=config.hord_tr)
-        self.tracer_advection = TracerAdvection(stencil_factory, tracer_transport, comm, NQ,
-                                                grid_data=grid_data)
+        self.tracer_advection = TracerAdvection(stencil_factory, tracer_transport, comm, NQ)
         self.acoustic_dynamics = AcousticDynamics(
             comm, stencil_factory, grid_data, damping_coefficients, config.grid_type, False,
             False, config.acoustic_dynamics, self._ak_host, self._bk_host, self._pfull,

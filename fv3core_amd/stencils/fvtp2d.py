@@ -60,6 +60,7 @@ class FiniteVolumeTransport:
     ):
         idx = stencil_factory.grid_indexing
         self._grid_indexing = idx
+        self.grid_data = grid_data
         self._area = grid_data.area
         self._dxa = grid_data.dxa
         self._dya = grid_data.dya

@@ -21,6 +21,10 @@ class MapSingle:
                  j1: int, j2: int):
         if kord > 10:
             raise AssertionError(f"kord {kord} not implemented.")
+        if abs(kord) > 10:
+            # remap_profile.py:611 runs other limiter forms for kord <= -11;
+            # those are not built here
+            raise NotImplementedError(f"kord {kord}: only -10 <= kord <= 10 is implemented")
         self._grid_indexing = stencil_factory.grid_indexing
         self._kord = int(kord)
         self._mode = int(mode)

@@ -33,9 +33,13 @@ class TracerAdvection:
         self._gi = gi
         self._tracer_count = tracer_count
         self.comm = comm
+        # the reference reads the module-level grid (spec.grid); here the
+        # metric terms come from the transport the caller hands in
         if grid_data is None:
-            raise ValueError("grid_data (GridData with dxa, dya, dx, dy, sin_sg1..4, rarea) "
-                             "is required")
+            grid_data = getattr(transport, "grid_data", None)
+        if grid_data is None:
+            raise ValueError("transport carries no grid_data (GridData with dxa, dya, dx, dy, "
+                             "sin_sg1..4, rarea)")
         self._grid = grid_data
         self._stencil_factory = stencil_factory
         self.finite_volume_transport = transport

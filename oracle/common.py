@@ -95,3 +95,15 @@ def transpose_ij(a):
     if a.ndim == 2:
         return a.T
     return np.swapaxes(a, 0, 1)
+
+
+def finite_on(a: np.ndarray, mask2: np.ndarray, what: str) -> np.ndarray:
+    """Return ``a`` after checking that it is finite wherever ``mask2`` (an IJ
+    mask) is set.  Expressions evaluated over the whole array divide by
+    zero-valued halo points outside the region a stencil writes; they are
+    computed under ``np.errstate`` and this check makes sure no inf or NaN
+    reaches the written region."""
+    bad = ~np.isfinite(a[mask2])
+    if bad.any():
+        raise FloatingPointError(f"{what}: {int(bad.sum())} non-finite values in the written region")
+    return a

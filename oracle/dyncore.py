@@ -12,7 +12,7 @@ import math
 
 import numpy as np
 
-from .common import box, sh, shk, sign
+from .common import box, finite_on, sh, shk, sign
 from .csw import fill_corners, put, region
 from .dsw import a2b_ord4
 from .fvtp2d import copy_corners_x, copy_corners_y, copy_corners_xy, delnflux_nosg, fvtp2d
@@ -171,20 +171,27 @@ def nh_p_grad(u, v, pp, gz, pk3, delp, dt, ptop, akap, g, t):
     gz0, gz1 = gz[..., K], gz[..., Kp]
     pk0, pk1 = pk3[..., K], pk3[..., Kp]
     pp0, pp1 = pp[..., K], pp[..., Kp]
-    du = dt / (w + sh(w, 1, 0)) * ((gz1 - sh(gz0, 1, 0)) * (sh(pk1, 1, 0) - pk0)
-                                   + (gz0 - sh(gz1, 1, 0)) * (pk1 - sh(pk0, 1, 0)))
-    un = (u[..., K] + du + dt / (w1 + sh(w1, 1, 0)) * (
-        (gz1 - sh(gz0, 1, 0)) * (sh(pp1, 1, 0) - pp0) + (gz0 - sh(gz1, 1, 0)) * (pp1 - sh(pp0, 1, 0))
-    )) * rdx
+    mu = box(s[:2], t.isc, t.iec, t.jsc, t.jec + 1)
+    mv = box(s[:2], t.isc, t.iec + 1, t.jsc, t.jec)
+    # a2b leaves wk1 at zero outside the B-grid box, so the divisions below
+    # are only defined on the boxes they are written to
+    with np.errstate(divide="ignore", invalid="ignore"):
+        du = dt / (w + sh(w, 1, 0)) * ((gz1 - sh(gz0, 1, 0)) * (sh(pk1, 1, 0) - pk0)
+                                       + (gz0 - sh(gz1, 1, 0)) * (pk1 - sh(pk0, 1, 0)))
+        un = (u[..., K] + du + dt / (w1 + sh(w1, 1, 0)) * (
+            (gz1 - sh(gz0, 1, 0)) * (sh(pp1, 1, 0) - pp0)
+            + (gz0 - sh(gz1, 1, 0)) * (pp1 - sh(pp0, 1, 0))
+        )) * rdx
+        dv = dt / (w + sh(w, 0, 1)) * ((gz1 - sh(gz0, 0, 1)) * (sh(pk1, 0, 1) - pk0)
+                                       + (gz0 - sh(gz1, 0, 1)) * (pk1 - sh(pk0, 0, 1)))
+        vn = (v[..., K] + dv + dt / (w1 + sh(w1, 0, 1)) * (
+            (gz1 - sh(gz0, 0, 1)) * (sh(pp1, 0, 1) - pp0)
+            + (gz0 - sh(gz1, 0, 1)) * (pp1 - sh(pp0, 0, 1))
+        )) * rdy
     uk = u[..., K]
-    put(uk, un, box(s[:2], t.isc, t.iec, t.jsc, t.jec + 1))
-    dv = dt / (w + sh(w, 0, 1)) * ((gz1 - sh(gz0, 0, 1)) * (sh(pk1, 0, 1) - pk0)
-                                   + (gz0 - sh(gz1, 0, 1)) * (pk1 - sh(pk0, 0, 1)))
-    vn = (v[..., K] + dv + dt / (w1 + sh(w1, 0, 1)) * (
-        (gz1 - sh(gz0, 0, 1)) * (sh(pp1, 0, 1) - pp0) + (gz0 - sh(gz1, 0, 1)) * (pp1 - sh(pp0, 0, 1))
-    )) * rdy
+    put(uk, finite_on(un, mu, "nh_p_grad u"), mu)
     vk = v[..., K]
-    put(vk, vn, box(s[:2], t.isc, t.iec + 1, t.jsc, t.jec))
+    put(vk, finite_on(vn, mv, "nh_p_grad v"), mv)
 
 
 # ------------------------------------------------------------ halo pressure

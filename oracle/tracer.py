@@ -11,7 +11,7 @@ import math
 
 import numpy as np
 
-from .common import sh
+from .common import box, finite_on, sh
 from .fvtp2d import Tile, copy_corners_xy, fvtp2d
 
 
@@ -56,6 +56,7 @@ def tracer_advection(tracers, dp1, mfxd, mfyd, cxd, cyd, metric, hord, halo, nz)
     dp2 = np.zeros(dp1.shape)
     a_dp1, a_dp2 = dp1, dp2  # storage identities, as in the reference
     ci, cj = slice(tile.isc, tile.iec + 1), slice(tile.jsc, tile.jec + 1)
+    cbox = box(cxd.shape[1:3], tile.isc, tile.iec, tile.jsc, tile.jec)
     for it in range(n_split):
         last = it == n_split - 1
         for t in range(ntile):
@@ -70,7 +71,11 @@ def tracer_advection(tracers, dp1, mfxd, mfyd, cxd, cyd, metric, hord, halo, nz)
                 fx, fy = fvtp2d(qx, qy, cxd[t], cyd[t], xfx[t], yfx[t], g["area"], g["dxa"],
                                 g["dya"], hord, tile, x_mass_flux=mfxd[t], y_mass_flux=mfyd[t])
                 ra = g["rarea"][:, :, None]
-                new = (q[t] * a_dp1[t] + (fx - sh(fx, 1, 0) + fy - sh(fy, 0, 1)) * ra) / a_dp2[t]
+                # dp2 is only formed on the compute box (zero in the halo)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    new = (q[t] * a_dp1[t] + (fx - sh(fx, 1, 0) + fy - sh(fy, 0, 1)) * ra) \
+                        / a_dp2[t]
+                finite_on(new[..., kz], cbox, "q_adjust")
                 q[t][ci, cj, kz] = new[ci, cj, kz]
         if not last:
             qs = halo(qs)

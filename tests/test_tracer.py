@@ -73,7 +73,7 @@ def test_gpu_tracer_advection_matches_oracle():
     grid = GridData({k: dev3(v) for k, v in metric.items()})
     comm = CubedSphereCommunicator(gi)
     transport = FiniteVolumeTransport(factory, grid, None, 0, 8)
-    ta = TracerAdvection(factory, transport, comm, len(qs), grid_data=grid)
+    ta = TracerAdvection(factory, transport, comm, len(qs))
     tq = [dev4(q) for q in qs]
     tdp, tmfx, tmfy, tcx, tcy = (dev4(a) for a in (dp, mfx, mfy, cx, cy))
     ta({f"q{i}": q for i, q in enumerate(tq)}, tdp, tmfx, tmfy, tcx, tcy, 225.0)
