@@ -229,13 +229,17 @@ def p_grad_c(uc, vc, delpc, pkc, gz, dt2, g, t, hydrostatic=False):
     gz0, gz1, pk0, pk1 = gz[..., K], gz[..., Kp], pkc[..., K], pkc[..., Kp]
     rdxc, rdyc = _g(g, "rdxc"), _g(g, "rdyc")
     m = box(uc.shape[:2], t.isc, t.iec + 1, t.jsc, t.jec + 1)
-    un = uc[..., K] + dt2 * rdxc / (sh(wk, -1, 0) + wk) * (
-        (sh(gz1, -1, 0) - gz0) * (pk1 - sh(pk0, -1, 0)) + (sh(gz0, -1, 0) - gz1) * (sh(pk1, -1, 0) - pk0))
-    vn = vc[..., K] + dt2 * rdyc / (sh(wk, 0, -1) + wk) * (
-        (sh(gz1, 0, -1) - gz0) * (pk1 - sh(pk0, 0, -1)) + (sh(gz0, 0, -1) - gz1) * (sh(pk1, 0, -1) - pk0))
+    # wk is only defined where delpc (pkc) was formed
+    with np.errstate(divide="ignore", invalid="ignore"):
+        un = uc[..., K] + dt2 * rdxc / (sh(wk, -1, 0) + wk) * (
+            (sh(gz1, -1, 0) - gz0) * (pk1 - sh(pk0, -1, 0))
+            + (sh(gz0, -1, 0) - gz1) * (sh(pk1, -1, 0) - pk0))
+        vn = vc[..., K] + dt2 * rdyc / (sh(wk, 0, -1) + wk) * (
+            (sh(gz1, 0, -1) - gz0) * (pk1 - sh(pk0, 0, -1))
+            + (sh(gz0, 0, -1) - gz1) * (sh(pk1, 0, -1) - pk0))
     uk, vk = uc[..., K], vc[..., K]
-    put(uk, un, m)
-    put(vk, vn, m)
+    put(uk, finite_on(un, m, "p_grad_c uc"), m)
+    put(vk, finite_on(vn, m, "p_grad_c vc"), m)
 
 
 # ------------------------------------------------------------------ ray_fast
@@ -395,10 +399,13 @@ def set_pem(delp, pem, ptop, t):
 
 
 # ----------------------------------------------------------- the driver
-def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, col, nz):
-    """AcousticDynamics.__call__ (dyn_core.py:479-761) on six tiles, in place.
+def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, col, nz,
+                      edges=True):
+    """AcousticDynamics.__call__ (dyn_core.py:479-761) on a batch of tiles, in place.
 
-    ``state``: dict of arrays [6, I, J, nz + 1] (IJ fields [6, I, J]),
+    ``edges``: the tile-edge flags (True for cube tiles; False for a doubly
+    periodic plane tile, whose ``halo`` wraps periodically).
+    ``state``: dict of arrays [T, I, J, nz + 1] (IJ fields [T, I, J]),
     including the component temporaries (ut, vt, gz, zh, pem, pkc, pk3,
     heat_source, divgd, crx, xfx, cry, yfx, ws3, delpc, ptc) as the
     reference keeps them in the state.  ``cfg``: dict of namelist values;
@@ -413,7 +420,8 @@ def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, c
     from .fvtp2d import Tile
 
     n = state["delp"].shape[1] - 7
-    t = Tile(n, n, 3, True, True, True, True)
+    ntiles = state["delp"].shape[0]
+    t = Tile(n, n, 3, edges, edges, edges, edges)
     K = slice(0, nz)
     Ki = slice(0, nz + 1)
     dt = cfg["mdt"] / cfg["n_split"]
@@ -442,13 +450,13 @@ def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, c
         remap_step = it == cfg["n_split"] - 1
         halo("scalar", ["w"])
         if it == 0:
-            for ti in range(6):
+            for ti in range(ntiles):
                 set_gz(zs[ti], S["delz"][ti], S["gz"][ti][..., Ki], t)
             halo("scalar", ["gz"])
         if it == cfg["n_split"] - 1 and end_step:
-            for ti in range(6):
+            for ti in range(ntiles):
                 set_pem(S["delp"][ti][..., K], S["pem"][ti][..., K], ptop, t)
-        for ti in range(6):
+        for ti in range(ntiles):
             g = metrics[ti]
             out = ocs.c_sw(*(S[k][ti][..., K] for k in ("delp", "pt", "u", "v", "w", "uc", "vc",
                                                          "ua", "va", "ut", "vt", "divgd",
@@ -460,14 +468,14 @@ def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, c
             S["ptc"][ti][..., K] = out["ptc"]
         if cfg["nord"] > 0:
             halo("corner", ["divgd"])
-        for ti in range(6):
+        for ti in range(ntiles):
             if it == 0:
                 S["zh"][ti][t.isd:t.ied + 1, t.jsd:t.jed + 1, Ki] = \
                     S["gz"][ti][t.isd:t.ied + 1, t.jsd:t.jed + 1, Ki]
             else:
                 S["gz"][ti][t.isd:t.ied + 1, t.jsd:t.jed + 1, Ki] = \
                     S["zh"][ti][t.isd:t.ied + 1, t.jsd:t.jed + 1, Ki]
-        for ti in range(6):
+        for ti in range(ntiles):
             g = metrics[ti]
             gz = S["gz"][ti][..., Ki]
             update_dz_c(dp_ref, zs[ti], g["area"], S["ut"][ti][..., Ki], S["vt"][ti][..., Ki],
@@ -483,7 +491,7 @@ def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, c
             p_grad_c(uc, vc, S["delpc"][ti][..., K], S["pkc"][ti][..., Ki], S["gz"][ti][..., Ki],
                      dt2, g, t)
         halo("vector_c", ["uc", "vc"])
-        for ti in range(6):
+        for ti in range(ntiles):
             g = metrics[ti]
             st = {"delpc": S["vt"][ti][..., K], "ptc": S["ptc"][ti][..., K]}
             for k in ("delp", "pt", "u", "v", "w", "uc", "vc", "ua", "va", "divgd", "q_con",
@@ -503,7 +511,7 @@ def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, c
             for sk, ok in back.items():
                 S[sk][ti][..., K] = out[ok]
         halo("scalar", ["delp", "pt", "q_con"])
-        for ti in range(6):
+        for ti in range(ntiles):
             g = metrics[ti]
             zh = S["zh"][ti][..., Ki]
             update_dz_d(zs[ti], zh, S["crx"][ti][..., Ki], S["cry"][ti][..., Ki],
@@ -519,7 +527,7 @@ def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, c
                 S[k][ti][c] = a
         halo("scalar", ["zh"])
         halo("scalar", ["pkc"], n_halo=2)
-        for ti in range(6):
+        for ti in range(ntiles):
             if remap_step:
                 pe_halo(S["pe"][ti][..., Ki], S["delp"][ti], ptop, t)
             pk3_halo(S["pk3"][ti][..., Ki], S["delp"][ti], ptop, akap, t)
@@ -537,7 +545,7 @@ def acoustic_dynamics(state, cfg, metrics, scalars, ak, bk, pfull, halo, sync, c
     if nk_heat != 0 and cfg["d_con"] > 1.0e-5:
         halo("scalar", ["heat_source"])
         cd = CNST_0P20 * scalars["da_min"]
-        for ti in range(6):
+        for ti in range(ntiles):
             g = metrics[ti]
             hs = S["heat_source"][ti][..., K]
             del2cubed(hs, cd, g, t, min(3, cfg["nord"] + 1))
