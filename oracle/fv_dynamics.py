@@ -47,10 +47,13 @@ def fv_setup_pt_adjust(S, t, nz):
     S["pt"][C] = S["pt"][C] * (1.0 + dp1) * (1.0 - q_con) / pkz
 
 
-def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, nz):
-    """One step_dynamics on six tiles, in place on ``S``.
+def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, nz,
+                edges=True):
+    """One step_dynamics on a batch of tiles, in place on ``S``.
 
-    ``S``: dict of [6, I, J, nz + 1] arrays (IJ fields [6, I, J]) holding the
+    ``edges``: the tile-edge flags (True for the cube tiles, False for a
+    doubly periodic plane tile whose halo functions wrap periodically).
+    ``S``: dict of [T, I, J, nz + 1] arrays (IJ fields [T, I, J]) holding the
     reference's ArgSpec fields plus the step and acoustic temporaries (cappa,
     dp1, cvm, wsd, te0_2d, ut, vt, gz, zh, pem, pkc, pk3, heat_source, divgd,
     crx, xfx, cry, yfx, ws3, delpc, ptc).  ``cfg``: namelist values plus bdt
@@ -60,7 +63,8 @@ def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, 
     D-grid edge winds.
     """
     n = S["delp"].shape[1] - 7
-    t = Tile(n, n, 3, True, True, True, True)
+    ntiles = S["delp"].shape[0]
+    t = Tile(n, n, 3, edges, edges, edges, edges)
     pfull = init_pfull(ak[:nz + 1], bk[:nz + 1], cfg.get("p_ref", 1.0e5))
     fv_setup_pt_adjust(S, t, nz)
     k_split = cfg["k_split"]
@@ -70,12 +74,13 @@ def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, 
         S["dp1"][...] = S["delp"]
         acfg = dict(cfg)
         acfg.update(mdt=cfg["bdt"] / k_split, n_map=n_map + 1)
-        acoustic_dynamics(S, acfg, metrics, scalars, ak, bk, pfull, halo, sync, col, nz)
+        acoustic_dynamics(S, acfg, metrics, scalars, ak, bk, pfull, halo, sync, col, nz,
+                          edges=edges)
         if cfg.get("z_tracer", True):
             metric6 = {k: np.stack([m[k] for m in metrics]) for k in metrics[0]}
             qs, dp1, mfxd, mfyd, cxd, cyd = tracer_advection(
                 [S[k] for k in tr_names], S["dp1"], S["mfxd"], S["mfyd"], S["cxd"], S["cyd"],
-                metric6, cfg["hord_tr"], tracer_halo, nz)
+                metric6, cfg["hord_tr"], tracer_halo, nz, edges=edges)
             for k, q in zip(tr_names, qs):
                 S[k][...] = q
             S["dp1"][...] = dp1
@@ -85,7 +90,7 @@ def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, 
                         kord_mt=cfg["kord_mt"], fill=cfg["fill"])
             names = ("pt", "delp", "delz", "peln", "u", "v", "w", "ua", "cappa", "q_con", "pkz",
                      "pk", "pe", "ps", "wsd")
-            for ti in range(6):
+            for ti in range(ntiles):
                 st = {k: S[k][ti] for k in names}
                 trs = {k: S[k][ti] for k in orm.TRACERS}
                 o, q = orm.lagrangian_to_eulerian(st, trs, ak, bk, cfg["ptop"], KAPPA, ZVIR,
@@ -100,6 +105,6 @@ def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, 
                 S["omga"][C] = S["delp"][C] / S["delz"][C] * S["w"][C]
                 if cfg.get("nf_omega", 1) > 0:
                     halo("scalar", ["omga"])
-                    for ti in range(6):
+                    for ti in range(ntiles):
                         om = S["omga"][ti][..., :nz]
                         del2cubed(om, 0.18 * scalars["da_min"], metrics[ti], t, cfg["nf_omega"])

@@ -28,7 +28,7 @@ def flux_compute(cx, cy, g, t: Tile):
     return xfx, yfx
 
 
-def tracer_advection(tracers, dp1, mfxd, mfyd, cxd, cyd, metric, hord, halo, nz):
+def tracer_advection(tracers, dp1, mfxd, mfyd, cxd, cyd, metric, hord, halo, nz, edges=True):
     """Returns (tracers, dp1_storage, mfxd, mfyd, cxd, cyd) after the call.
 
     ``metric``: dict of [6, I, J] arrays; ``halo(list_of_fields)`` returns the
@@ -36,7 +36,7 @@ def tracer_advection(tracers, dp1, mfxd, mfyd, cxd, cyd, metric, hord, halo, nz)
     """
     ntile = cxd.shape[0]
     n = cxd.shape[1] - 7
-    tile = Tile(n, n, 3, True, True, True, True)
+    tile = Tile(n, n, 3, edges, edges, edges, edges)
     qs = [np.array(q, copy=True) for q in tracers]
     dp1 = np.array(dp1, copy=True)
     mfxd, mfyd, cxd, cyd = (np.array(a, copy=True) for a in (mfxd, mfyd, cxd, cyd))

@@ -189,7 +189,7 @@ def test_acoustic_step_keeps_a_resting_isothermal_cube_at_rest():
 
 
 # --------------------------------------------------------------- plane states
-def _plane_state(n, seed, symmetric):
+def _plane_state(n, seed, symmetric, temperature=False):
     """A moving, stratified periodic plane state: the hybrid column of a
     lapse-rate atmosphere with smooth periodic perturbations of pt and delp
     and smooth periodic winds.  With ``symmetric`` the scalars are symmetric
@@ -234,6 +234,10 @@ def _plane_state(n, seed, symmetric):
     for k in range(NZ - 1, -1, -1):
         S["zh"][..., k] = S["zh"][..., k + 1] - S["delz"][..., k]
     S["pkz"][..., :NZ] = np.diff(S["pk"], axis=3) / (KAPPA * np.diff(S["peln"], axis=3))
+    if temperature:
+        # DynamicalCore takes temperature and forms the potential temperature
+        # itself (fv_setup + pt_adjust)
+        S["pt"][..., :NZ] *= np.exp(KAPPA * np.log(pm))
     return S, col
 
 
@@ -286,3 +290,129 @@ def test_acoustic_step_keeps_mirror_symmetry():
     np.testing.assert_allclose(u, np.swapaxes(v, 0, 1), rtol=1e-11, atol=1e-11 * np.abs(u).max())
     # and the state did move
     assert np.abs(S["w"][c]).max() > 1e-6
+
+
+# ------------------------------------------------------------ the full step
+def _full_step(S, cfg, col, m, ntiles, n, edges, halo_cls):
+    """oracle.fv_dynamics.fv_dynamics on S (tracers included), in place."""
+    from oracle import fv_dynamics as ofd
+
+    shape = S["delp"].shape
+    S["phis"] = np.zeros(shape[:3])
+    S["ps"] = S["pe"][..., NZ].copy()
+    for k in ("wsd", "te0_2d"):
+        S[k] = np.zeros(shape[:3])
+    for k in ("cappa", "dp1", "cvm"):
+        S[k] = np.zeros(shape)
+    _add_temporaries(S)
+    S["zh"] = S["zh"].copy()
+    halo = halo_cls(S, n)
+
+    def tracer_halo(qs):
+        out = []
+        for q in qs:
+            box = {"_q": np.array(q, copy=True)}
+            halo_cls(box, n)("scalar", ["_q"])
+            out.append(box["_q"])
+        return out
+
+    ocfg = _oracle_cfg(cfg, col["ptop"])
+    ocfg.update(bdt=float(cfg.dt_atmos), hord_tr=cfg.hord_tr, kord_tm=cfg.kord_tm,
+                kord_tr=cfg.kord_tr, kord_wz=cfg.kord_wz, kord_mt=cfg.kord_mt, fill=cfg.fill,
+                z_tracer=cfg.z_tracer, nf_omega=cfg.nf_omega, p_ref=cfg.p_ref)
+    ofd.fv_dynamics(S, ocfg, _metric_list(m, ntiles),
+                    dict(da_min=m["da_min"], da_min_c=m["da_min_c"]), col["ak"], col["bk"],
+                    halo, tracer_halo, halo.sync,
+                    get_column_namelist(cfg.d_grid_shallow_water, NZ), NZ, edges=edges)
+
+
+def _step_cfg():
+    return DynamicalCoreConfig(npz=NZ, n_split=2, k_split=1, nord=3, dt_atmos=225,
+                               kord_tm=-9, kord_tr=9, kord_wz=9, kord_mt=9)
+
+
+def _add_tracers(S, n, rng, symmetric):
+    from oracle.remapping import TRACERS
+
+    shape = S["delp"].shape
+    x = (np.arange(n + 7) - 3 + 0.5) / n
+    lev = np.linspace(0.3, 1.0, shape[-1])[None, None, None, :]
+    for name in TRACERS:
+        ph = rng.uniform(0, 2 * np.pi, size=2)
+        f = 1.0 + 0.8 * np.sin(2 * np.pi * x[:, None] + ph[0]) * np.cos(2 * np.pi * x[None, :] + ph[1])
+        if symmetric:
+            f = 0.5 * (f + f.T)
+        scale = 8.0e-3 if name == "qvapor" else 2.0e-5
+        S[name] = np.array(np.broadcast_to(scale * f[None, :, :, None] * lev, shape))
+        for t in range(shape[0]):
+            S[name][t] = fill_periodic(S[name][t], n)
+
+
+def test_full_step_conserves_mass_and_tracer_mass_on_a_periodic_plane():
+    """One DynamicalCore step (acoustic substeps, 8-tracer sub-cycled
+    transport, kord-9 remap with fillz, omega) on a doubly periodic moist
+    plane keeps sum(delp * area) and, for each transported tracer,
+    sum(q * delp * area) to roundoff."""
+    from oracle.remapping import TRACERS
+
+    n = 8
+    S, col = _plane_state(n, seed=23, symmetric=False, temperature=True)
+    _add_tracers(S, n, np.random.default_rng(24), symmetric=False)
+    c = (0, slice(3, n + 3), slice(3, n + 3), slice(0, NZ))
+    mass0 = S["delp"][c].sum()
+    tmass0 = {k: (S[k][c] * S["delp"][c]).sum() for k in TRACERS[:8]}
+    q0 = S["qrain"][c].copy()
+    m = plane_metric(n, f=1.0e-4)
+    _full_step(S, _step_cfg(), col, m, 1, n, False, _PeriodicHalo)
+    np.testing.assert_allclose(S["delp"][c].sum(), mass0, rtol=1e-13)
+    for k in TRACERS[:8]:
+        np.testing.assert_allclose((S[k][c] * S["delp"][c]).sum(), tmass0[k], rtol=1e-12,
+                                   err_msg=k)
+    assert np.abs(S["qrain"][c] - q0).max() > 1e-3 * np.abs(q0).max(), "tracers should move"
+
+
+def test_full_step_keeps_mirror_symmetry():
+    """A mirror-symmetric moist plane state stays mirror-symmetric through a
+    whole step, tracers and remap included."""
+    from oracle.remapping import TRACERS
+
+    n = 8
+    S, col = _plane_state(n, seed=25, symmetric=True, temperature=True)
+    _add_tracers(S, n, np.random.default_rng(26), symmetric=True)
+    _full_step(S, _step_cfg(), col, plane_metric(n, f=0.0), 1, n, False, _PeriodicHalo)
+    c = (0, slice(3, n + 3), slice(3, n + 3), slice(0, NZ))
+    for name in ("delp", "pt", "delz", "w") + TRACERS[:8]:
+        a = S[name][c]
+        np.testing.assert_allclose(a, np.swapaxes(a, 0, 1), rtol=1e-10,
+                                   atol=1e-11 * np.abs(a).max(), err_msg=name)
+    cu = (0, slice(3, n + 3), slice(3, n + 4), slice(0, NZ))
+    cv = (0, slice(3, n + 4), slice(3, n + 3), slice(0, NZ))
+    np.testing.assert_allclose(S["u"][cu], np.swapaxes(S["v"][cv], 0, 1), rtol=1e-10,
+                               atol=1e-11 * np.abs(S["u"][cu]).max())
+
+
+def test_full_step_keeps_a_resting_cube_at_rest():
+    """Six C8 tiles, resting isothermal dry atmosphere (DynamicalCore takes
+    temperature) carrying horizontally uniform profiles of the two passive
+    tracers: one whole step (fv_setup, acoustic substeps, tracer transport,
+    remap, omega) leaves delp, delz, the temperature and the tracers
+    unchanged to roundoff and the winds at roundoff."""
+    from oracle.remapping import TRACERS
+
+    n = 8
+    col = _column(NZ)
+    shape = (6, n + 7, n + 7, NZ + 1)
+    S = _state_from_column(col, shape, NZ)
+    S["pt"][..., :NZ] *= np.exp(KAPPA * np.log(col["pm"]))[None, None, None, :]
+    lev = np.linspace(0.3, 1.0, NZ + 1)
+    for name in TRACERS:
+        passive = name in ("qo3mr", "qsgs_tke")
+        S[name] = np.array(np.broadcast_to((1.0e-4 * lev) if passive else 0.0 * lev, shape))
+    S0 = {k: v.copy() for k, v in S.items()}
+    m = cube_state(n, NZ, seed=0)["metric"]
+    _full_step(S, _step_cfg(), col, m, 6, n, True, _CubeHalo)
+    c = (slice(None), slice(3, n + 3), slice(3, n + 3), slice(0, NZ))
+    for name in ("delp", "delz", "pt", "qo3mr", "qsgs_tke"):
+        np.testing.assert_allclose(S[name][c], S0[name][c], rtol=1e-12, err_msg=name)
+    assert np.abs(S["u"][(slice(None), slice(3, n + 3), slice(3, n + 4), slice(0, NZ))]).max() < 1e-9
+    assert np.abs(S["w"][c]).max() < 1e-9
