@@ -368,3 +368,71 @@ def test_gpu_multistep_transport_remap_gradient(hord):
         print(f"hord={hord} dir {seed}: J={cost:.6e} FD {fd:.10e} adjoint {ad:.10e} "
               f"rel {_rel(fd, ad):.1e}")
         assert _rel(fd, ad) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# torch-autodiff reference for the map_scalar adjoint (tests/torch_remap.py)
+# ---------------------------------------------------------------------------
+
+
+def _remap_columns(n, nz, mode, workload=4):
+    """Columns [ncol, km] of the remap workload's compute box, as numpy."""
+    from fv3core_amd.workloads import remap_inputs
+    d = remap_inputs(n, nz, workload=workload)
+    h = N_HALO
+    box = (slice(h, h + n), slice(h, h + n))
+    field = d["w"] if mode in (-2, 1, 2) else d["pt"]
+    q = field[box][..., :nz].reshape(n * n, nz)
+    pe1 = d["pe1"][box].reshape(n * n, nz + 1)
+    pe2 = d["pe2"][box].reshape(n * n, nz + 1)
+    qs = d["qs"][box].reshape(n * n)
+    return d, q, pe1, pe2, qs
+
+
+@pytest.mark.parametrize("mode", [-2, -1, 0, 1, 2])
+def test_torch_remap_restatement_matches_oracle(mode):
+    """The torch restatement of map_scalar (kord 9) equals the numpy oracle's
+    MapSingle on the remap workload's columns for every iv mode."""
+    from oracle import remap as orp
+    from tests import torch_remap as tr
+    n, nz = 6, 79
+    d, q, pe1, pe2, qs = _remap_columns(n, nz, mode)
+    got = tr.map_scalar(torch.as_tensor(q), pe1, pe2, mode,
+                        qs=qs if mode == -2 else None).numpy()
+    h = N_HALO
+    field = d["w"] if mode in (-2, 1, 2) else d["pt"]
+    want = orp.map_single(field, d["pe1"], d["pe2"], 9, mode, h, h + n - 1, h, h + n - 1,
+                          qs=d["qs"] if mode == -2 else None)
+    want = want[h:h + n, h:h + n, :nz].reshape(n * n, nz)
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [-2, 0, 1])
+def test_gpu_remap_adjoint_matches_torch_autodiff(mode):
+    """The hand-written reverse sweep of map_scalar (kord 9) against torch
+    autograd of the torch restatement, on the workload's tie-free columns:
+    g_q = (d remap / d q)^T g_out to 1e-10."""
+    from tests import torch_remap as tr
+    n, nz = 16, 79
+    d, dev, lin = _remap_setup(n, nz, mode)
+    q = _ijk(_remap_field(dev, mode))
+    g = torch.Generator(device="cpu").manual_seed(70 + mode)
+    gout = _ijk(torch.randn(q.shape, generator=g, dtype=q.dtype).cuda())
+    qs = dev["qs"] if mode == -2 else None
+    gin = lin.adjoint(q, gout, dev["pe1"], dev["pe2"], qs)
+    torch.cuda.synchronize()
+    h = N_HALO
+    box = (slice(h, h + n), slice(h, h + n))
+    qc = torch.tensor(q.cpu().numpy()[box][..., :nz].reshape(n * n, nz), requires_grad=True)
+    pe1 = d["pe1"][box].reshape(n * n, nz + 1)
+    pe2 = d["pe2"][box].reshape(n * n, nz + 1)
+    out = tr.map_scalar(qc, pe1, pe2, mode,
+                        qs=d["qs"][box].reshape(n * n) if mode == -2 else None)
+    go = torch.as_tensor(gout.cpu().numpy()[box][..., :nz].reshape(n * n, nz))
+    (out * go).sum().backward()
+    want = qc.grad.numpy()
+    got = gin.cpu().numpy()[box][..., :nz].reshape(n * n, nz)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"mode={mode}: max |adjoint - autodiff| / max|autodiff| = {err:.2e}")
+    assert err < 1e-10

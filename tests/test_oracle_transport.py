@@ -70,6 +70,25 @@ def test_xppm_linear_field_is_exact_upwind_mean(iord):
     np.testing.assert_allclose(f[sl], (expect * np.ones(_shape()))[sl], rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize("iord", ORDS)
+def test_xppm_linear_field_is_exact_on_the_tile_edges(iord):
+    # With the cube-edge anchors on, the one-sided face values at i_start,
+    # i_start +- 1, i_end, i_end + 1, i_end + 2 (xppm.py:105-246: the
+    # xt_dxa_edge extrapolations and the c1/c2/c3 edge weights) are exact for
+    # a linear field on a uniform grid, like the interior 4th-order faces:
+    # every face of the compute domain gives the exact upwind mean
+    a, b = 0.37, -2.0
+    ii = np.arange(_shape()[0])[:, None, None]
+    q = (a * ii + b) * np.ones(_shape())
+    rng = np.random.default_rng(2)
+    c = rng.uniform(-0.95, 0.95, _shape())
+    dxa = np.ones(_shape()[:2])
+    f = xppm(q, c, dxa, iord, CUBE.i_start, CUBE.i_end)
+    expect = (a * (ii - 0.5) + b) - c * a / 2
+    sl = (slice(CUBE.isc, CUBE.iec + 2),)
+    np.testing.assert_allclose(f[sl], (expect * np.ones(_shape()))[sl], rtol=1e-13, atol=1e-14)
+
+
 @pytest.mark.parametrize("iord", [5, 6, 7])
 def test_xppm_zero_courant_gives_face_value(iord):
     # c = 0 takes the downwind branch with fx1 = bl: the result is al, the
