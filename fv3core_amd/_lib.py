@@ -93,6 +93,10 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+# return codes of the C ABI (include/fv3core_amd.h)
+E_ARG, E_UNSUPPORTED, E_HIP = -1, -2, -3
+
+
 def check(rc: int) -> None:
     if rc != 0:
         msg = lib().fv3_last_error().decode()

@@ -240,6 +240,9 @@ struct ColumnPPM {
 // the host.  Sources are never corner cells.  present = 0: q_y is read as it
 // is.  (Coefficients are picked with constant indices so that the table stays
 // in the kernel arguments.)
+// Most scalars one transport launch carries (the tracers of one call).
+constexpr int kMaxScalarBatch = FV3_MAX_TRACERS;
+
 struct YFill {
   int present;  // bit c: corner c is filled
   int ai[4], bi[4], ci[4], aj[4], bj[4], cj[4];
@@ -247,6 +250,28 @@ struct YFill {
 
 __device__ __forceinline__ int sel4(int c, int v0, int v1, int v2, int v3) {
   return c == 0 ? v0 : c == 1 ? v1 : c == 2 ? v2 : v3;
+}
+
+// Scalars transported by one launch with the same Courant numbers, area and
+// mass fluxes (the tracers of TracerAdvection): each slot's storage and flux
+// outputs.  The plain call is a batch of one.
+template <typename S>
+struct ScalarBatch {
+  int n;
+  const S* qx[kMaxScalarBatch];
+  const S* qy[kMaxScalarBatch];
+  S* fx[kMaxScalarBatch];
+  S* fy[kMaxScalarBatch];
+};
+
+// a[b] for a wave-uniform slot b, as a chain of selects (no indexed access
+// to the kernel arguments)
+template <typename P>
+__device__ __forceinline__ P pick_slot(const P (&a)[kMaxScalarBatch], int b) {
+  P r = a[0];
+#pragma unroll
+  for (int m = 1; m < kMaxScalarBatch; ++m) r = (b == m) ? a[m] : r;
+  return r;
 }
 
 // S is the storage type of every field, T the type the march computes in
@@ -392,21 +417,25 @@ __device__ __forceinline__ void fvtp2d_strip(
 
 template <typename S, typename T, int ORD>
 __global__ void __launch_bounds__(MW)
-    fvtp2d_kernel(Dom d, const S* __restrict__ q_x, const S* __restrict__ q_y,
-                  const S* __restrict__ crx, const S* __restrict__ cry,
-                  const S* __restrict__ xfx, const S* __restrict__ yfx,
-                  const S* __restrict__ xmf, const S* __restrict__ ymf,
-                  const S* __restrict__ area, const S* __restrict__ dxa,
-                  const S* __restrict__ dya, S* __restrict__ fx, S* __restrict__ fy,
-                  int nbi, int nbj, YFill yf) {
+    fvtp2d_kernel(Dom d, ScalarBatch<S> sb, const S* __restrict__ crx,
+                  const S* __restrict__ cry, const S* __restrict__ xfx,
+                  const S* __restrict__ yfx, const S* __restrict__ xmf,
+                  const S* __restrict__ ymf, const S* __restrict__ area,
+                  const S* __restrict__ dxa, const S* __restrict__ dya, int nbi, int nbj,
+                  YFill yf) {
   const int per_plane = nbi * nbj;
-  const int total = per_plane * d.nk * d.ntiles;
+  const int total = per_plane * d.nk * d.ntiles * sb.n;
   const int lb = xcd_logical_block(blockIdx.x, total);
   if (lb >= total) return;
   const int plane = lb / per_plane;
   const int within = lb % per_plane;
   const int bi = within % nbi, bj = within / nbi;
-  const int k = plane % d.nk, t = plane / d.nk;
+  const int k = plane % d.nk, tb = plane / d.nk;
+  const int t = tb % d.ntiles, b = tb / d.ntiles;
+  const S* __restrict__ q_x = pick_slot(sb.qx, b);
+  const S* __restrict__ q_y = pick_slot(sb.qy, b);
+  S* __restrict__ fx = pick_slot(sb.fx, b);
+  S* __restrict__ fy = pick_slot(sb.fy, b);
   const int i0 = d.isc + bi * MTI;
   const int j0 = d.jsc + bj * MTJ;
   const int64_t off3 = (int64_t)t * d.st + (int64_t)k * d.sk;
@@ -436,18 +465,21 @@ __global__ void __launch_bounds__(MW)
 #undef FV3_TP_STRIP
 }
 
-// T: storage type; C: arithmetic type of the kernel.
+// T: storage type; C: arithmetic type of the kernel.  sb: the scalars
+// transported with the shared winds and fluxes (one for the plain call).
 template <typename T, typename C = T>
-static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T* q_y,
-                         const T* crx, const T* cry, const T* xfx, const T* yfx,
-                         const T* xmf, const T* ymf, const T* area, const T* dxa,
-                         const T* dya, T* fx, T* fy, void* stream,
-                         const YFill* yfill = nullptr) {
+static int fvtp2d_batch_launch(const fv3_domain* dom, int hord, const ScalarBatch<T>& sb,
+                               const T* crx, const T* cry, const T* xfx, const T* yfx,
+                               const T* xmf, const T* ymf, const T* area, const T* dxa,
+                               const T* dya, void* stream, const YFill* yfill = nullptr) {
   const char* who = "fv3_fvtp2d";
   int rc = check_domain(dom, who);
   if (rc) return rc;
-  FV3_REQUIRE(q_x && q_y && crx && cry && xfx && yfx && area && dxa && dya && fx && fy,
-              FV3_E_ARG, "%s: required pointer is NULL", who);
+  FV3_REQUIRE(sb.n >= 1 && sb.n <= kMaxScalarBatch, FV3_E_ARG,
+              "%s: %d scalars in one launch (1 .. %d)", who, sb.n, kMaxScalarBatch);
+  bool have = crx && cry && xfx && yfx && area && dxa && dya;
+  for (int b = 0; b < sb.n; ++b) have = have && sb.qx[b] && sb.qy[b] && sb.fx[b] && sb.fy[b];
+  FV3_REQUIRE(have, FV3_E_ARG, "%s: required pointer is NULL", who);
   FV3_REQUIRE(hord == 5 || hord == 6 || hord == 7 || hord == 8, FV3_E_UNSUPPORTED,
               "%s: hord=%d not supported (5, 6, 7, 8)", who, hord);
   if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
@@ -459,7 +491,7 @@ static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T*
   Dom d = to_dom(dom);
   const int nfi = d.iec + 2 - d.isc, nfj = d.jec + 2 - d.jsc;
   const int nbi = (nfi + MTI - 1) / MTI, nbj = (nfj + MTJ - 1) / MTJ;
-  const int64_t total = (int64_t)nbi * nbj * d.nk * d.ntiles;
+  const int64_t total = (int64_t)nbi * nbj * d.nk * d.ntiles * sb.n;
   FV3_REQUIRE(total < (int64_t)1 << 30, FV3_E_ARG, "%s: grid too large", who);
   const int64_t launched = (total + kNumXcd - 1) / kNumXcd * kNumXcd;
   dim3 grid((unsigned)launched), block(MW);
@@ -469,30 +501,49 @@ static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T*
   // algorithmic bytes: q (and its y-corner copy when it is a separate
   // array), crx, cry, both area fluxes and the optional mass fluxes read
   // once, both flux fields written once, over the compute cells
-  const double nfld = 7.0 + (q_y != q_x ? 1.0 : 0.0) + (xmf ? 1.0 : 0.0) + (ymf ? 1.0 : 0.0);
+  // (per scalar: q, its y-corner copy when that is a separate array, both
+  // fluxes; shared: crx, cry, both area fluxes, the optional mass fluxes)
+  double nfld = 4.0 + (xmf ? 1.0 : 0.0) + (ymf ? 1.0 : 0.0);
+  for (int b = 0; b < sb.n; ++b) nfld += 3.0 + (sb.qy[b] != sb.qx[b] ? 1.0 : 0.0);
   TimedLaunch timed("fvtp2d_kernel", s,
                     nfld * sizeof(T) * (double)(d.iec - d.isc + 1) * (d.jec - d.jsc + 1) *
                         d.nk * d.ntiles);
   switch (hord) {
     case 5:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 5>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj, yf);
+      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 5>), grid, block, 0, s, d, sb, crx, cry, xfx, yfx,
+                         xmf, ymf, area, dxa, dya, nbi, nbj, yf);
       break;
     case 6:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 6>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj, yf);
+      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 6>), grid, block, 0, s, d, sb, crx, cry, xfx, yfx,
+                         xmf, ymf, area, dxa, dya, nbi, nbj, yf);
       break;
     case 7:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 7>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj, yf);
+      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 7>), grid, block, 0, s, d, sb, crx, cry, xfx, yfx,
+                         xmf, ymf, area, dxa, dya, nbi, nbj, yf);
       break;
     default:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 8>), grid, block, 0, s, d, q_x, q_y, crx, cry,
-                         xfx, yfx, xmf, ymf, area, dxa, dya, fx, fy, nbi, nbj, yf);
+      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 8>), grid, block, 0, s, d, sb, crx, cry, xfx, yfx,
+                         xmf, ymf, area, dxa, dya, nbi, nbj, yf);
       break;
   }
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
+}
+
+template <typename T, typename C = T>
+static int fvtp2d_launch(const fv3_domain* dom, int hord, const T* q_x, const T* q_y,
+                         const T* crx, const T* cry, const T* xfx, const T* yfx,
+                         const T* xmf, const T* ymf, const T* area, const T* dxa,
+                         const T* dya, T* fx, T* fy, void* stream,
+                         const YFill* yfill = nullptr) {
+  ScalarBatch<T> sb{};
+  sb.n = 1;
+  sb.qx[0] = q_x;
+  sb.qy[0] = q_y;
+  sb.fx[0] = fx;
+  sb.fy[0] = fy;
+  return fvtp2d_batch_launch<T, C>(dom, hord, sb, crx, cry, xfx, yfx, xmf, ymf, area, dxa, dya,
+                                   stream, yfill);
 }
 
 // ---------------------------------------------------------------------------
@@ -708,30 +759,23 @@ static int corners_launch(const fv3_domain* dom, const T* q, T* q_x, T* q_y, voi
 // also an x destination and no source is a corner cell, so q after its x
 // fill, read with the y fill, is exactly the copy the two-step form builds.
 // When the two destination sets differ, the copy is built in qy_scratch.
-template <typename T, typename C>
-static int fvtp2d_corner_fill_launch(const fv3_domain* dom, int hord, T* q, T* qy_scratch,
-                                     const T* crx, const T* cry, const T* xfx, const T* yfx,
-                                     const T* xmf, const T* ymf, const T* area, const T* dxa,
-                                     const T* dya, T* fx, T* fy, void* stream) {
-  const char* who = "fv3_fvtp2d";
-  int rc = check_domain(dom, who);
-  if (rc) return rc;
-  FV3_REQUIRE(q && qy_scratch, FV3_E_ARG, "%s: required pointer is NULL", who);
-  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+// The x corner table of the domain and, when every y destination is also an
+// x destination and each corner's 3 x 3 y square is an affine map of its
+// destinations, the fit of those maps (yf); returns false otherwise.
+static bool fit_corner_fill(const fv3_domain* dom, CornerTable* xt, YFill* yf_out) {
   CornerTable tab;
   build_corner_table(dom, 3, &tab);
-  CornerTable xt;
-  xt.n = 0;
+  xt->n = 0;
   int ny = 0;
   int ydi[72], ydj[72], ysi[72], ysj[72];
   for (int e = 0; e < tab.n; ++e) {
     if (tab.dir[e] == 0) {
-      const int m = xt.n++;
-      xt.dst_i[m] = tab.dst_i[e];
-      xt.dst_j[m] = tab.dst_j[e];
-      xt.src_i[m] = tab.src_i[e];
-      xt.src_j[m] = tab.src_j[e];
-      xt.dir[m] = 0;
+      const int m = xt->n++;
+      xt->dst_i[m] = tab.dst_i[e];
+      xt->dst_j[m] = tab.dst_j[e];
+      xt->src_i[m] = tab.src_i[e];
+      xt->src_j[m] = tab.src_j[e];
+      xt->dir[m] = 0;
     } else {
       ydi[ny] = tab.dst_i[e];
       ydj[ny] = tab.dst_j[e];
@@ -742,10 +786,11 @@ static int fvtp2d_corner_fill_launch(const fv3_domain* dom, int hord, T* q, T* q
   }
   // every y destination must be an x destination and vice versa, and each
   // corner's 9 y entries an exact affine map of the destination
-  bool same = xt.n == ny;
+  bool same = xt->n == ny;
   for (int a = 0; same && a < ny; ++a) {
     bool hit = false;
-    for (int b = 0; b < xt.n; ++b) hit = hit || (xt.dst_i[b] == ydi[a] && xt.dst_j[b] == ydj[a]);
+    for (int b = 0; b < xt->n; ++b)
+      hit = hit || (xt->dst_i[b] == ydi[a] && xt->dst_j[b] == ydj[a]);
     same = hit;
   }
   YFill yf{};
@@ -803,8 +848,42 @@ static int fvtp2d_corner_fill_launch(const fv3_domain* dom, int hord, T* q, T* q
     yf.cj[c] = cJ;
     counted += 9;
   }
-  same = same && counted == ny;
-  if (!same) {
+  *yf_out = yf;
+  return same && counted == ny;
+}
+
+// x corner fill in place of every scalar of a batch: blockIdx.z is
+// tile + ntiles * slot.
+template <typename T>
+__global__ void corners_x_batch_kernel(Dom d, CornerTable tab, ScalarBatch<T> sb) {
+  const int e = threadIdx.x;
+  if (e >= tab.n) return;
+  const int t = (int)blockIdx.z % d.ntiles, b = (int)blockIdx.z / d.ntiles;
+  T* q = const_cast<T*>(pick_slot(sb.qx, b));
+  const int64_t base = (int64_t)t * d.st + (int64_t)blockIdx.y * d.sk;
+  q[base + (int64_t)tab.dst_j[e] * d.sj + tab.dst_i[e]] =
+      q[base + (int64_t)tab.src_j[e] * d.sj + tab.src_i[e]];
+}
+
+// fv_tp_2d of q preceded by the reference's corner copies: copy_corners_x in
+// place on q (as CopyCornersXY with q_x = q) and copy_corners_y applied as the
+// march reads q, instead of into a full copy of q.  Every y destination is
+// also an x destination and no source is a corner cell, so q after its x
+// fill, read with the y fill, is exactly the copy the two-step form builds.
+// When the two destination sets differ, the copy is built in qy_scratch.
+template <typename T, typename C>
+static int fvtp2d_corner_fill_launch(const fv3_domain* dom, int hord, T* q, T* qy_scratch,
+                                     const T* crx, const T* cry, const T* xfx, const T* yfx,
+                                     const T* xmf, const T* ymf, const T* area, const T* dxa,
+                                     const T* dya, T* fx, T* fy, void* stream) {
+  const char* who = "fv3_fvtp2d";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(q && qy_scratch, FV3_E_ARG, "%s: required pointer is NULL", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  CornerTable xt;
+  YFill yf{};
+  if (!fit_corner_fill(dom, &xt, &yf)) {
     rc = corners_launch<T>(dom, q, q, qy_scratch, stream);
     if (rc) return rc;
     return fvtp2d_launch<T, C>(dom, hord, q, qy_scratch, crx, cry, xfx, yfx, xmf, ymf, area,
@@ -818,6 +897,41 @@ static int fvtp2d_corner_fill_launch(const fv3_domain* dom, int hord, T* q, T* q
   }
   return fvtp2d_launch<T, C>(dom, hord, q, q, crx, cry, xfx, yfx, xmf, ymf, area, dxa, dya, fx,
                              fy, stream, &yf);
+}
+
+int fvtp2d_batch_corner_fill_f64(const fv3_domain* dom, int hord, int n, double* const* q,
+                                 double* const* fx, double* const* fy, const double* crx,
+                                 const double* cry, const double* xfx, const double* yfx,
+                                 const double* xmf, const double* ymf, const double* area,
+                                 const double* dxa, const double* dya, void* stream) {
+  const char* who = "fv3_fvtp2d";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(n >= 1 && n <= kMaxScalarBatch, FV3_E_ARG, "%s: %d scalars (1 .. %d)", who, n,
+              kMaxScalarBatch);
+  FV3_REQUIRE(q && fx && fy, FV3_E_ARG, "%s: NULL pointer table", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  CornerTable xt;
+  YFill yf{};
+  FV3_REQUIRE(fit_corner_fill(dom, &xt, &yf), FV3_E_UNSUPPORTED,
+              "%s: the batched transport needs the x and y corner sets of a whole cube tile "
+              "(or no corners); transport the scalars one by one", who);
+  ScalarBatch<double> sb{};
+  sb.n = n;
+  for (int b = 0; b < n; ++b) {
+    FV3_REQUIRE(q[b] && fx[b] && fy[b], FV3_E_ARG, "%s: NULL pointer in slot %d", who, b);
+    sb.qx[b] = sb.qy[b] = q[b];
+    sb.fx[b] = fx[b];
+    sb.fy[b] = fy[b];
+  }
+  if (xt.n > 0) {
+    Dom d = to_dom(dom);
+    hipLaunchKernelGGL((corners_x_batch_kernel<double>), dim3(1, d.nk, d.ntiles * n), dim3(128),
+                       0, as_stream(stream), d, xt, sb);
+    FV3_HIP_CHECK_LAUNCH(who);
+  }
+  return fvtp2d_batch_launch<double, double>(dom, hord, sb, crx, cry, xfx, yfx, xmf, ymf, area,
+                                             dxa, dya, stream, &yf);
 }
 
 int fvtp2d_corner_fill_f64(const fv3_domain* dom, int hord, double* q, double* qy_scratch,

@@ -27,4 +27,15 @@ int fvtp2d_corner_fill_f64(const fv3_domain* dom, int hord, double* q, double* q
                            const double* area, const double* dxa, const double* dya, double* fx,
                            double* fy, bool fp32_arithmetic, void* stream);
 
+// fv_tp_2d of n scalars (<= FV3_MAX_TRACERS) sharing the Courant numbers,
+// area and mass fluxes, in one launch after one launch of their x corner
+// fills in place (the corner handling of fvtp2d_corner_fill_f64).  q, fx, fy:
+// host tables of n device pointers.  FV3_E_UNSUPPORTED when the domain's x
+// and y corner sets differ (then the scalars go one by one).
+int fvtp2d_batch_corner_fill_f64(const fv3_domain* dom, int hord, int n, double* const* q,
+                                 double* const* fx, double* const* fy, const double* crx,
+                                 const double* cry, const double* xfx, const double* yfx,
+                                 const double* xmf, const double* ymf, const double* area,
+                                 const double* dxa, const double* dya, void* stream);
+
 }  // namespace fv3

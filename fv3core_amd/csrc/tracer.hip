@@ -6,6 +6,7 @@
 // and q_adjust.  flux_compute and cmax_multiply_by_frac are fused into one
 // pass over the storage; the transport itself is fv3_fvtp2d.
 #include "common.hpp"
+#include "kernels.hpp"
 
 namespace fv3 {
 
@@ -74,9 +75,74 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// q_adjust of a batch of tracers: blockIdx.z = tile + ntiles * slot
+struct TracerPtrs {
+  double* q[FV3_MAX_TRACERS];
+  const double* fx[FV3_MAX_TRACERS];
+  const double* fy[FV3_MAX_TRACERS];
+};
+
+__global__ void __launch_bounds__(256)
+    tracer_adjust_batch_kernel(Dom d, TracerPtrs p, const double* __restrict__ dp1,
+                               const double* __restrict__ rarea,
+                               const double* __restrict__ dp2) {
+  const int nci = d.iec - d.isc + 1, ncj = d.jec - d.jsc + 1;
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)nci * ncj) return;
+  const int i = d.isc + (int)(n % nci), j = d.jsc + (int)(n / nci);
+  const int k = blockIdx.y, t = (int)blockIdx.z % d.ntiles, b = (int)blockIdx.z / d.ntiles;
+  double* q = p.q[0];
+  const double *fx = p.fx[0], *fy = p.fy[0];
+#pragma unroll
+  for (int m = 1; m < FV3_MAX_TRACERS; ++m) {
+    q = b == m ? p.q[m] : q;
+    fx = b == m ? p.fx[m] : fx;
+    fy = b == m ? p.fy[m] : fy;
+  }
+  const int64_t o = (int64_t)t * d.st + (int64_t)k * d.sk + (int64_t)j * d.sj + i;
+  const double ra = rarea[(int64_t)t * d.st2 + (int64_t)j * d.sj2 + i];
+  const double div = (fx[o] - fx[o + 1] + fy[o] - fy[o + d.sj]) * ra;
+  q[o] = (q[o] * dp1[o] + div) / dp2[o];
+}
+
 }  // namespace fv3
 
 extern "C" {
+
+int32_t fv3_tracer_transport_f64(const fv3_domain* dom, int32_t hord, int32_t ntracers,
+                                 double* const* q, double* const* fx, double* const* fy,
+                                 const double* crx, const double* cry, const double* xfx,
+                                 const double* yfx, const double* mfx, const double* mfy,
+                                 const double* area, const double* dxa, const double* dya,
+                                 const double* rarea, const double* dp1, const double* dp2,
+                                 void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_tracer_transport";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(ntracers >= 1 && ntracers <= FV3_MAX_TRACERS, FV3_E_ARG,
+              "%s: %d tracers (1 .. %d)", who, ntracers, FV3_MAX_TRACERS);
+  FV3_REQUIRE(q && fx && fy && crx && cry && xfx && yfx && mfx && mfy && area && dxa && dya &&
+                  rarea && dp1 && dp2,
+              FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  rc = fvtp2d_batch_corner_fill_f64(dom, hord, ntracers, q, fx, fy, crx, cry, xfx, yfx, mfx,
+                                    mfy, area, dxa, dya, stream);
+  if (rc) return rc;
+  TracerPtrs p{};
+  for (int b = 0; b < ntracers; ++b) {
+    p.q[b] = q[b];
+    p.fx[b] = fx[b];
+    p.fy[b] = fy[b];
+  }
+  Dom d = to_dom(dom);
+  const int64_t ncell = (int64_t)(d.iec - d.isc + 1) * (d.jec - d.jsc + 1);
+  dim3 grid((unsigned)((ncell + 255) / 256), d.nk, d.ntiles * ntracers);
+  hipLaunchKernelGGL(tracer_adjust_batch_kernel, grid, dim3(256), 0, as_stream(stream), d, p,
+                     dp1, rarea, dp2);
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
 
 int32_t fv3_tracer_flux_prep_f64(const fv3_domain* dom, int32_t n_split, double* cxd,
                                  double* cyd, double* mfxd, double* mfyd, double* xfx,

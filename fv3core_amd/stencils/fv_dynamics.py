@@ -77,8 +77,17 @@ class DynamicalCore:
     """Fortran fv_dynamics."""
 
     def __init__(self, comm, grid_data: GridData, stencil_factory: StencilFactory,
-                 damping_coefficients: DampingCoefficients, config, ak, bk, phis):
+                 damping_coefficients: DampingCoefficients, config, ak, bk, phis,
+                 use_graph: bool = False):
+        """``use_graph`` (no counterpart in the reference): capture a whole
+        step (every k_split iteration of the acoustic substeps, tracer
+        transport and remap) in one HIP graph and replay it on later steps
+        with the same storages and scalars; the first step for a new set runs
+        eagerly, the second captures and replays."""
         gi = stencil_factory.grid_indexing
+        self._use_graph = bool(use_graph)
+        self._graphs = {}
+        self._seen_keys = set()
         if not config.moist_phys:
             raise AssertionError("fvsetup is only implemented for moist_phys=true")
         if config.nwat != 6:
@@ -182,7 +191,28 @@ class DynamicalCore:
             "mdt": timestep / self.config.k_split, "do_adiabatic_init": do_adiabatic_init,
             "ptop": ptop, "n_split": n_split, "k_split": self.config.k_split, "ks": ks,
         })
-        self._compute(ns, timer)
+        if not self._use_graph:
+            self._compute(ns, timer)
+            return
+        key = (tuple((n, ns.__dict__[n].data_ptr()) for n in ARG_NAMES if n not in ("ak", "bk")),
+               float(timestep), float(ptop), int(n_split), int(ks), bool(do_adiabatic_init),
+               float(conserve_total_energy))
+        graph = self._graphs.get(key)
+        if graph is not None:
+            graph.replay()
+            return
+        if key not in self._seen_keys:
+            # first step for these storages: eager, fills every cache
+            self._seen_keys.add(key)
+            self._compute(ns, timer)
+            return
+        graph = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph):
+            self._compute(ns, NullTimer())
+        torch.cuda.synchronize()
+        self._graphs[key] = graph
+        graph.replay()
 
     def _compute(self, state, timer):
         if self._temporaries is None:

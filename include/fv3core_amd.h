@@ -311,6 +311,27 @@ int32_t fv3_tracer_adjust_f64(const fv3_domain* d, int32_t mode, double* out,
                               const double* rarea, const double* dp2, void* stream);
 
 /*
+ * One sub-cycle of TracerAdvection's loop over tracers
+ * (fv3core/stencils/tracer_2d_1l.py:264-296) for ntracers (1 .. FV3_MAX_TRACERS)
+ * tracers at once: the x corner copies of every tracer in place, fv_tp_2d
+ * of every tracer with the shared Courant numbers and area fluxes and the
+ * mass fluxes mfx, mfy (hord; the y corner copies applied as the transport
+ * reads q, as fv3_fvtp2d_corner_fill_f64 does), then q_adjust
+ * q = (q dp1 + div(fx, fy) rarea) / dp2 on the compute domain, in three
+ * launches.  q, fx, fy: host arrays of ntracers device pointers (fx, fy are
+ * flux scratch storages laid out like q).  FV3_E_UNSUPPORTED when the
+ * domain's x and y corner sets differ (partial tile edges): then transport
+ * the tracers one by one with fv3_fvtp2d.
+ */
+int32_t fv3_tracer_transport_f64(const fv3_domain* d, int32_t hord, int32_t ntracers,
+                                 double* const* q, double* const* fx, double* const* fy,
+                                 const double* crx, const double* cry, const double* xfx,
+                                 const double* yfx, const double* mfx, const double* mfy,
+                                 const double* area, const double* dxa, const double* dya,
+                                 const double* rarea, const double* dp1, const double* dp2,
+                                 void* stream);
+
+/*
  * IJ metric terms of GridData (fv3core/utils/grid.py:474-798), as device
  * pointers of the entry point's floating type, each laid out with the
  * domain's sj2/st2 strides.  Members an entry point does not read may be

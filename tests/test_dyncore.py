@@ -97,7 +97,19 @@ def test_sync_map_joins_the_edge_winds():
 
 
 @pytest.mark.gpu
-def test_gpu_acoustic_dynamics_matches_oracle():
+@pytest.mark.parametrize("nz,n_split", [(NZ, 2), (79, 6)])
+def test_gpu_acoustic_dynamics_matches_oracle(nz, n_split):
+    """One AcousticDynamics call on six C12 tiles against the oracle driver:
+    8 synthetic levels with 2 substeps, and the reference's 79-level hybrid
+    grid (eta.py) with n_split = 6, where the sponge-layer damping columns
+    (n_sponge, the k-banded nord / damping coefficients of
+    get_column_namelist) and the Rayleigh damping all act.
+
+    Tolerances come from the oracle's own sensitivity: rerunning it with the
+    results of exp and log moved by one ulp at random (what different math
+    libraries do) moves w by up to 4.9e-6 and u, v by up to 1.0e-9 in the
+    79-level case (pt, delp, pe below 1e-15, delz and zh near 1e-12);
+    the GPU is allowed 10-20x that."""
     from fv3core_amd.halo import CubedSphereCommunicator
     from fv3core_amd.stencils.d_sw import get_column_namelist
     from fv3core_amd.stencils.dyn_core import AcousticDynamics
@@ -105,12 +117,13 @@ def test_gpu_acoustic_dynamics_matches_oracle():
     from fv3core_amd.utils.grid import DampingCoefficients, GridData
     from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
 
+    NZ = nz  # noqa: N806 (the case's level count)
     init = cube_state(N, NZ, seed=2)
     st0 = init["state"]
     m = init["metric"]
     ak, bk, ptop = init["ak"], init["bk"], init["ptop"]
     pfull = init_pfull(ak, bk)
-    cfg = _config()
+    cfg = DynamicalCoreConfig(npz=NZ, n_split=n_split, k_split=1, nord=3, dt_atmos=225)
     gi = GridIndexing((N, N, NZ), 3, True, True, True, True)
     factory = StencilFactory(StencilConfig(), gi)
 
@@ -157,15 +170,18 @@ def test_gpu_acoustic_dynamics_matches_oracle():
                for t in range(6)]
     odc.acoustic_dynamics(S, ocfg, metrics, dict(da_min=m["da_min"], da_min_c=m["da_min_c"]),
                           ak, bk, pfull, halo, halo.sync, col, NZ)
+    tol = dict(TOL) if nz < 79 else {"w": 1.0e-4, "omga": 1.0e-4, "u": 1.0e-8, "v": 1.0e-8}
     c = (slice(None), slice(3, N + 3), slice(3, N + 3), slice(0, NZ))
     for name in ("delp", "pt", "w", "delz", "q_con", "pkz"):
         got = getattr(state, name).cpu().numpy()
-        assert_close(got[c], S[name][c], TOL.get(name, 1e-9), near_zero=1e-12, what=name,
+        assert_close(got[c], S[name][c], tol.get(name, 1e-9), near_zero=1e-12, what=name,
                      scale_floor=1e-6)
     cu = (slice(None), slice(3, N + 3), slice(3, N + 4), slice(0, NZ))
     cv = (slice(None), slice(3, N + 4), slice(3, N + 3), slice(0, NZ))
-    assert_close(state.u.cpu().numpy()[cu], S["u"][cu], 1e-9, what="u", scale_floor=1e-6)
-    assert_close(state.v.cpu().numpy()[cv], S["v"][cv], 1e-9, what="v", scale_floor=1e-6)
+    assert_close(state.u.cpu().numpy()[cu], S["u"][cu], tol.get("u", 1e-9), what="u",
+                 scale_floor=1e-6)
+    assert_close(state.v.cpu().numpy()[cv], S["v"][cv], tol.get("v", 1e-9), what="v",
+                 scale_floor=1e-6)
     ci = (slice(None), slice(3, N + 3), slice(3, N + 3), slice(0, NZ + 1))
     for name in ("pe", "peln", "pk", "zh"):
         got = getattr(state, name).cpu().numpy()

@@ -85,3 +85,42 @@ def test_gpu_tracer_advection_matches_oracle():
     assert_close(tdp.cpu().numpy()[c], ref[1][c], 1e-12, what="dp1")
     for g, r, name in zip((tmfx, tmfy, tcx, tcy), ref[2:], ("mfxd", "mfyd", "cxd", "cyd")):
         assert_close(g.cpu().numpy()[..., :NZ], r[..., :NZ], 1e-15, what=name)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ntr", [1, 8])
+def test_gpu_batched_tracer_transport_equals_one_by_one(ntr):
+    """The batched sub-cycle transport (one x corner fill, one fv_tp_2d and
+    one q update launch for every tracer) returns the bits of the
+    per-tracer loop (corner copies, fv_tp_2d, q update for each tracer)."""
+    from fv3core_amd.halo import CubedSphereCommunicator
+    from fv3core_amd.stencils.fvtp2d import FiniteVolumeTransport
+    from fv3core_amd.stencils.tracer_2d_1l import TracerAdvection
+    from fv3core_amd.utils import gt4py_utils as utils
+    from fv3core_amd.utils.grid import GridData
+    from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
+
+    qs, dp, mfx, mfy, cx, cy, metric = _inputs(3, ntr=ntr)
+    gi = GridIndexing((N, N, NZ), 3, True, True, True, True)
+    factory = StencilFactory(StencilConfig(), gi)
+
+    def dev(a, bd=None):
+        t = utils.empty(a.shape, batch_dims=bd)
+        t.copy_(torch.as_tensor(a))
+        return t
+
+    grid = GridData({k: dev(v, 1) for k, v in metric.items()})
+    comm = CubedSphereCommunicator(gi)
+    runs = []
+    for batched in (True, False):
+        transport = FiniteVolumeTransport(factory, grid, None, 0, 8)
+        ta = TracerAdvection(factory, transport, comm, ntr)
+        assert ta._batched
+        ta._batched = batched
+        tq = [dev(q) for q in qs]
+        args = [dev(a) for a in (dp, mfx, mfy, cx, cy)]
+        ta({f"q{i}": q for i, q in enumerate(tq)}, *args, 225.0)
+        torch.cuda.synchronize()
+        runs.append(tq + args)
+    for a, b in zip(*runs):
+        assert torch.equal(a, b)
