@@ -4,10 +4,15 @@ One call of ``step_dynamics`` runs the reference's sequence on the batched
 six-tile storages: the preamble (fv_setup + pt_adjust, one fused kernel),
 then k_split times the copy delp -> dp1, AcousticDynamics, TracerAdvection
 and LagrangianToEulerian, and after the last remap set_omega, the omega
-halo update and HyperdiffusionDamping of omga.  The wrap-up the reference
-runs at the end of the step (AdjustNegativeTracerMixingRatio and
-CubedToLatLon, fv_dynamics.py:136-169) is not part of this path and is not
-run: qcld, ua and va are left as the remap leaves them.
+halo update and HyperdiffusionDamping of omga.
+
+The wrap-up the reference runs at the end of the step (fv_dynamics.py:
+136-169) is not built: AdjustNegativeTracerMixingRatio (neg_adj3), which
+corrects negative qvapor, qliquid, qrain, qsnow, qice, qgraupel and qcld
+(moving water between species and adjusting pt), and CubedToLatLon, which
+recomputes ua and va from the D-grid winds.  A caller must say so by
+constructing DynamicalCore with ``skip_wrapup=True``; the step then leaves
+those nine fields as the remap leaves them.
 """
 from __future__ import annotations
 
@@ -78,12 +83,18 @@ class DynamicalCore:
 
     def __init__(self, comm, grid_data: GridData, stencil_factory: StencilFactory,
                  damping_coefficients: DampingCoefficients, config, ak, bk, phis,
-                 use_graph: bool = False):
+                 use_graph: bool = False, skip_wrapup: bool = False):
         """``use_graph`` (no counterpart in the reference): capture a whole
         step (every k_split iteration of the acoustic substeps, tracer
         transport and remap) in one HIP graph and replay it on later steps
         with the same storages and scalars; the first step for a new set runs
         eagerly, the second captures and replays."""
+        if not skip_wrapup:
+            raise NotImplementedError(
+                "the step's wrap-up (fv_dynamics.py:136-169) is not implemented: "
+                "AdjustNegativeTracerMixingRatio would correct negative qvapor, qliquid, qrain, "
+                "qsnow, qice, qgraupel and qcld and adjust pt, CubedToLatLon would recompute ua "
+                "and va; pass skip_wrapup=True to step without it")
         gi = stencil_factory.grid_indexing
         self._use_graph = bool(use_graph)
         self._graphs = {}

@@ -216,7 +216,7 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord):
     damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
     comm = CubedSphereCommunicator(gi)
     phis = dev3(init["phis"])
-    core = DynamicalCore(comm, grid, factory, damp, cfg, ak, bk, phis)
+    core = DynamicalCore(comm, grid, factory, damp, cfg, ak, bk, phis, skip_wrapup=True)
     state = {k: dev4(v) for k, v in st0.items() if k not in ("cappa",)}
     state.update({k: dev4(v) for k, v in tr0.items()})
     state["phis"] = phis
@@ -302,7 +302,7 @@ def test_gpu_dynamical_core_graph_replay_matches_eager():
     cores, states = [], []
     for use_graph in (False, True):
         cores.append(DynamicalCore(comm, grid, factory, damp, cfg, init["ak"], init["bk"], phis,
-                                   use_graph=use_graph))
+                                   use_graph=use_graph, skip_wrapup=True))
         st = {k: dev(v, None) for k, v in init["state"].items() if k != "cappa"}
         st.update({k: dev(v, None) for k, v in init["tracers"].items()})
         st["phis"] = phis
