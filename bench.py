@@ -60,7 +60,9 @@ def _parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=None)
     p.add_argument("--warmup", type=int, default=None)
-    p.add_argument("--workload", default="dyncore",
+    p.add_argument("--no-adjoint", action="store_true",
+                   help="ensemble: skip the adjoint/forward ratio run")
+    p.add_argument("--workload", default="ensemble",
                    choices=["dyncore", "fv_dynamics", "fvtp2d", "adjoint", "ensemble"])
     p.add_argument("--roofline-kernel", default="fvtp2d_kernel",
                    choices=["fvtp2d_kernel", "riem3_kernel", "riem_c_kernel",
@@ -108,9 +110,9 @@ def cpu_baseline(seconds: float):
     }
 
 
-def cpu_baseline_dyncore(seconds: float):
-    """Oracle AcousticDynamics (numpy, FP64, one core) on six C24 tiles x 79
-    levels, n_split = 2: a bounded sample of the C96 workload."""
+def _oracle_acoustic_sample(seconds: float):
+    """(median seconds per call, calls, cell-level updates per call) of the
+    oracle AcousticDynamics on six C24 tiles x 79 levels, n_split = 2."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     import warnings
 
@@ -147,17 +149,44 @@ def cpu_baseline_dyncore(seconds: float):
                                                          da_min_c=m["da_min_c"]),
                                   ak, bk, pf, halo, halo.sync, col, nz)
         times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    cells = 6 * n * n * nz * n_split
-    return {
+    return float(np.median(times)), len(times), 6 * n * n * nz * n_split
+
+
+def _oracle_worker(seconds):
+    return _oracle_acoustic_sample(seconds)
+
+
+def cpu_baseline_dyncore(seconds: float, all_cores: bool = True):
+    """Oracle AcousticDynamics (numpy, FP64) on six C24 tiles x 79 levels,
+    n_split = 2 (a bounded sample of the acoustic workloads): one core, and
+    with ``all_cores`` one independent member per worker process on up to 16
+    cores at once (members need no exchange, as in the batched ensemble).
+    Runs before the process touches the GPU (the pool forks)."""
+    import multiprocessing
+
+    med, calls, cells = _oracle_acoustic_sample(seconds)
+    out = {
         "value": cells / med,
         "unit": "cell-level-updates/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"numpy oracle AcousticDynamics on 6 x C{n} x {nz} levels, n_split={n_split}, "
-        f"same initial state and namelist, median of {len(times)} calls "
-        f"({med:.2f} s/call), host {os.cpu_count()} cpus visible",
+        "sample": f"numpy oracle AcousticDynamics on 6 x C24 x 79 levels, n_split=2, "
+        f"same initial state and namelist, median of {calls} calls ({med:.2f} s/call), "
+        f"host {os.cpu_count()} cpus visible",
     }
+    if all_cores:
+        workers = max(1, min(16, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        with multiprocessing.get_context("fork").Pool(workers) as pool:
+            res = pool.map(_oracle_worker, [seconds] * workers)
+        elapsed = time.perf_counter() - t0
+        rate = sum(c / m for m, _, c in res)
+        out["all_cores"] = {
+            "value": rate, "unit": "cell-level-updates/s", "cores": workers,
+            "sample": f"{workers} worker processes, each timing its own member of the same "
+            f"sample concurrently (median per worker), {elapsed:.1f} s wall",
+        }
+    return out
 
 
 def _copy_calibration():
@@ -504,29 +533,33 @@ def bench_fvtp2d(steps: int, warmup: int, dtype_name: str, copy_calibration: boo
     return out
 
 
-def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str):
-    """Batched ensemble: 32 members of six C48 tiles x 79 levels (the C96
-    workload's initial state at C48, each member with seeded N(0, 0.5 K) pt
-    and N(0, 1 m/s) u, v perturbations) in one batch of 192 tiles, plus four
-    uneven members at C24, C64, C96 and C128, each its own batch; FP64.  A
-    step is one AcousticDynamics call (n_split = 6, dt = 225 s) of every
-    batch; cell-level updates per step = sum over members of 6 n^2 79 6."""
-    import torch
+# algorithmic HBM bytes per cell and acoustic substep of the whole step
+# (every state field read and written once, the accumulators read and
+# written, the C-grid winds written): 40 FP64 transfers
+STEP_BYTES_PER_CELL_FP64 = 320.0
 
-    from fv3core_amd import _lib
-    from fv3core_amd._config import DynamicalCoreConfig
-    from fv3core_amd.ensemble import UnevenEnsemble, ensemble_acoustic
+
+def _ensemble_parts(cfg, mixed):
+    from fv3core_amd.ensemble import ensemble_acoustic
     from fv3core_amd.workloads import cube_state
 
-    nz, n_split = 79, 6
-    cfg = DynamicalCoreConfig(npz=nz, n_split=n_split, k_split=1, nord=3, dt_atmos=225)
     plan = [(48, 32)] + [(n, 1) for n in (24, 64, 96, 128)]
     parts, cells = [], 0
     for n, members in plan:
-        init = cube_state(n, nz, seed=0)
-        parts.append(ensemble_acoustic(init, members, cfg, seed=1000 * n))
-        cells += members * 6 * n * n * nz * n_split
+        init = cube_state(n, cfg.npz, seed=0)
+        parts.append(ensemble_acoustic(init, members, cfg, seed=1000 * n,
+                                       mixed_precision=mixed))
+        cells += members * 6 * n * n * cfg.npz * cfg.n_split
         del init
+    return parts, cells
+
+
+def _time_ensemble(parts, steps, warmup, roofline_kernel, graph):
+    import torch
+
+    from fv3core_amd import _lib
+    from fv3core_amd.ensemble import UnevenEnsemble
+
     ens = UnevenEnsemble(parts)
     lib = _lib.lib()
     for _ in range(warmup):
@@ -553,22 +586,112 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str):
             ens()
         torch.cuda.synchronize()
         graph_wall = time.perf_counter() - t0
+    nz = parts[0][1].pt.shape[-1] - 1
     finite = all(bool(torch.isfinite(st.pt[:, 3:-4, 3:-4, :nz]).all().item()) for _, st in parts)
+    return wall, graph_wall, cnt, kern_ms, kern_bytes, finite
+
+
+def _ensemble_mixed_errors(cfg):
+    """Errors of the mixed form against the FP64 form after one call on the
+    uneven C96 member (same initial state, same perturbation)."""
+    import torch
+
+    from fv3core_amd.ensemble import ensemble_acoustic, member_slice
+    from fv3core_amd.workloads import cube_state
+
+    init = cube_state(96, cfg.npz, seed=0)
+    built = {}
+
+    def make(mixed):
+        acoustic, state = ensemble_acoustic(init, 1, cfg, seed=96000, mixed_precision=mixed)
+        built[mixed] = state
+        return acoustic
+
+    plain, mixed = make(False), make(True)
+    sp, sm = built[False], member_slice(built[False], 0)
+    plain(sp)
+    mixed(sm)
+    torch.cuda.synchronize()
+    n, nz = 96, cfg.npz
+    c = (slice(None), slice(3, n + 3), slice(3, n + 3))
+
+    def rel(a, b):
+        scale = float(torch.maximum(a.abs().max(), b.abs().max()))
+        den = torch.clamp(a.abs() + b.abs(), min=1e-6 * scale + 1e-300)
+        return float((2.0 * (a - b).abs() / den).max())
+
+    errs = {k: rel(getattr(sm, k)[c + (slice(0, nz),)], getattr(sp, k)[c + (slice(0, nz),)])
+            for k in ("delp", "pt", "delz", "w", "u", "v", "zh", "pe", "pk", "peln", "pkz", "ua",
+                      "va", "uc", "vc", "omga")}
+    errs["ps"] = rel(sm.pe[c + (nz,)], sp.pe[c + (nz,)])
+    return errs
+
+
+def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
+                   adjoint: bool = True):
+    """Batched ensemble: 32 members of six C48 tiles x 79 levels (the C96
+    workload's initial state at C48, each member with seeded N(0, 0.5 K) pt
+    and N(0, 1 m/s) u, v perturbations) in one batch of 192 tiles, plus four
+    uneven members at C24, C64, C96 and C128, each its own batch.  A step is
+    one AcousticDynamics call (n_split = 6, dt = 225 s) of every batch;
+    cell-level updates per step = sum over members of 6 n^2 79 6.
+
+    Both forms are timed: plain FP64 and the mixed form (FP32 arithmetic in
+    d_sw's fv_tp_2d transports, FP64 storages, every other kernel FP64).
+    ``value`` is the mixed form's rate, with the FP64 rate, their ratio, the
+    mixed form's per-field error against the FP64 form (one call on the C96
+    member) and, unless ``adjoint`` is off, the adjoint/forward time ratio of
+    the transport + remap adjoint workload beside it."""
+    import gc
+
+    import torch
+
+    from fv3core_amd._config import DynamicalCoreConfig
+
+    nz, n_split = 79, 6
+    cfg = DynamicalCoreConfig(npz=nz, n_split=n_split, k_split=1, nord=3, dt_atmos=225)
+    errs = _ensemble_mixed_errors(cfg)
+    results = {}
+    cells = 0
+    for form in ("fp64", "mixed"):
+        parts, cells = _ensemble_parts(cfg, form == "mixed")
+        results[form] = _time_ensemble(parts, steps, warmup, roofline_kernel,
+                                       graph and form == "mixed")
+        del parts
+        gc.collect()
+        torch.cuda.empty_cache()
+    wall64, _, _, _, _, finite64 = results["fp64"]
+    wall, graph_wall, cnt, kern_ms, kern_bytes, finite = results["mixed"]
+    used = graph_wall if graph_wall else wall
     avg_s = kern_ms / max(cnt, 1) * 1e-3
     achieved = kern_bytes / max(kern_ms * 1e-3, 1e-30) / 1e9
-    used = graph_wall if graph_wall else wall
-    return {
+    step_bytes = cells * STEP_BYTES_PER_CELL_FP64
+    out = {
         "value": steps * cells / used,
         "ms_per_step": used / steps * 1e3,
         "eager_ms_per_step": wall / steps * 1e3,
         "graph_ms_per_step": graph_wall / steps * 1e3 if graph_wall else None,
+        "fp64_value": steps * cells / wall64,
+        "fp64_ms_per_step": wall64 / steps * 1e3,
+        "mixed_over_fp64": wall64 / used,
+        "mixed_vs_fp64": errs,
+        "mixed_gate_ok": max(errs[k] for k in ("ps", "pt", "delp")) <= 1e-5,
+        "step_roofline": {
+            "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
+            "bytes_per_cell_substep": STEP_BYTES_PER_CELL_FP64,
+            "achieved": step_bytes / (used / steps) / 1e9,
+            "frac": step_bytes / (used / steps) / 1e9 / HBM_PEAK_GBPS,
+            "fp64_achieved": step_bytes / (wall64 / steps) / 1e9,
+        },
         "config": {
             "workload": "batched ensemble: 32 members x 6 x C48 x 79 in one batch + uneven "
             "members C24/C64/C96/C128 (one cube each), AcousticDynamics n_split=6, dt=225 s, "
-            "perturbed jet states, FP64" + (", each batch replayed as a HIP graph" if graph
-                                            else ""),
+            "perturbed jet states; mixed form: FP32 arithmetic in d_sw's fv_tp_2d transports, "
+            "FP64 storages, riem solvers, remap and every other kernel FP64" +
+            (", each batch replayed as a HIP graph" if graph else ""),
             "members": 36, "tiles": 6 * 36, "nz": nz, "n_split": n_split,
-            "cells_per_step": cells, "state_finite_after_timed_steps": finite,
+            "cells_per_step": cells,
+            "state_finite_after_timed_steps": bool(finite and finite64),
         },
         "roofline": {
             "bound": "hbm",
@@ -582,8 +705,13 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str):
             "kernel_avg_us": avg_s * 1e6,
             "algorithmic_bytes_per_launch": kern_bytes / max(cnt, 1),
         },
-        "dtype": "f64",
+        "dtype": "f64+f32",
     }
+    if adjoint:
+        adj = bench_adjoint(3, 1)
+        out["adjoint_over_forward"] = adj["adjoint_over_forward"]
+        out["adjoint_workload"] = adj["config"]["workload"]
+    return out
 
 
 def bench_adjoint(steps: int, warmup: int):
@@ -691,9 +819,9 @@ def main():
     if args.workload == "dyncore":
         steps = 20 if args.steps is None else args.steps
         warmup = 3 if args.warmup is None else args.warmup
+        cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
         res = bench_dyncore(steps, warmup, args.roofline_kernel, args.graph,
                             args.copy_calibration, args.mixed)
-        cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
     elif args.workload == "fv_dynamics":
         steps = 5 if args.steps is None else args.steps
         warmup = 1 if args.warmup is None else args.warmup
@@ -702,8 +830,9 @@ def main():
     elif args.workload == "ensemble":
         steps = 3 if args.steps is None else args.steps
         warmup = 1 if args.warmup is None else args.warmup
-        res = bench_ensemble(steps, warmup, args.graph, args.roofline_kernel)
-        cpu = None
+        cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
+        res = bench_ensemble(steps, warmup, args.graph, args.roofline_kernel,
+                             adjoint=not args.no_adjoint)
     elif args.workload == "adjoint":
         steps = 10 if args.steps is None else args.steps
         warmup = 2 if args.warmup is None else args.warmup
@@ -732,8 +861,9 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": cpu,
     }
-    for extra in ("eager_ms_per_step", "graph_ms_per_step", "adjoint_over_forward",
-                  "reverse_ms_per_step", "mixed_vs_fp64", "mixed_gate_ok"):
+    for extra in ("eager_ms_per_step", "graph_ms_per_step", "fp64_value", "fp64_ms_per_step",
+                  "mixed_over_fp64", "step_roofline", "adjoint_over_forward",
+                  "adjoint_workload", "reverse_ms_per_step", "mixed_vs_fp64", "mixed_gate_ok"):
         if res.get(extra) is not None:
             line[extra] = res[extra]
     if res.get("copy_calibration_bytes"):

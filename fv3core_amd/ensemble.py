@@ -43,7 +43,8 @@ def replicate(t: torch.Tensor, members: int) -> torch.Tensor:
 
 
 def ensemble_acoustic(init: Dict, members: int, cfg: DynamicalCoreConfig, seed: int = 0,
-                      pt_sd: float = 0.5, uv_sd: float = 1.0, use_graph: bool = False):
+                      pt_sd: float = 0.5, uv_sd: float = 1.0, use_graph: bool = False,
+                      mixed_precision: bool = False):
     """(AcousticDynamics, state) for ``members`` perturbed copies of a cube state.
 
     ``init`` is what :func:`fv3core_amd.workloads.cube_state` returns.  Member
@@ -87,7 +88,7 @@ def ensemble_acoustic(init: Dict, members: int, cfg: DynamicalCoreConfig, seed: 
     comm.get_vector_halo_updater().update([state.u], [state.v])
     acoustic = AcousticDynamics(comm, factory, grid, damp, 0, False, False, cfg.acoustic_dynamics,
                                 init["ak"], init["bk"], init_pfull(init["ak"], init["bk"], 1.0e5),
-                                phis, use_graph=use_graph)
+                                phis, use_graph=use_graph, mixed_precision=mixed_precision)
     return acoustic, state
 
 
