@@ -167,6 +167,8 @@ class DynamicalCoreConfig:
     nf_omega: int = 1
     fv_sg_adj: int = 600
     n_sponge: int = 48
+    check_negative: bool = False
+    c2l_ord: int = 4
 
     @property
     def riemann(self) -> RiemannConfig:

@@ -230,6 +230,30 @@ def tile_metrics(n: int, n_halo: int = 3, radius: float = RADIUS) -> Dict[str, n
     ag1[:, :nc, :nc] = np.arctan2(C[..., 1], C[..., 0])
     ag2[:, :nc, :nc] = np.arcsin(np.clip(C[..., 2], -1.0, 1.0))
     m["agrid1"], m["agrid2"] = ag1, ag2
+    # cell-centre transform from the covariant cube winds to east/north
+    # winds (fv3core/grid/geometry.py:14-61 get_center_vector, :888-905
+    # unit_vector_lonlat, :587-601 calculate_grid_z / calculate_grid_a):
+    # ec1 / ec2 are the unit tangents at the centre along the great circles
+    # through the west/east and south/north edge midpoints, and
+    # a = 0.5 * inverse of [[ec1.vlon, ec1.vlat], [ec2.vlon, ec2.vlat]]
+    # with determinant sin_sg5 = |ec1 x ec2|.
+    ec1 = _normalize(np.cross(C, np.cross(me, mw)))
+    ec2 = _normalize(np.cross(C, np.cross(mn, ms)))
+    lon = np.arctan2(C[..., 1], C[..., 0])
+    lat = np.arcsin(np.clip(C[..., 2], -1.0, 1.0))
+    zero = np.zeros_like(lon)
+    vlon = np.stack([-np.sin(lon), np.cos(lon), zero], axis=-1)
+    vlat = np.stack([-np.sin(lat) * np.cos(lon), -np.sin(lat) * np.sin(lon), np.cos(lat)], axis=-1)
+    z11 = np.sum(ec1 * vlon, axis=-1)
+    z12 = np.sum(ec1 * vlat, axis=-1)
+    z21 = np.sum(ec2 * vlon, axis=-1)
+    z22 = np.sum(ec2 * vlat, axis=-1)
+    cos5 = np.sum(ec1 * ec2, axis=-1)
+    sin5 = np.sqrt(np.maximum(0.0, 1.0 - cos5 ** 2))
+    m["a11"] = pad(0.5 * z22 / sin5, 0.0)
+    m["a12"] = pad(-0.5 * z12 / sin5, 0.0)
+    m["a21"] = pad(-0.5 * z21 / sin5, 0.0)
+    m["a22"] = pad(0.5 * z11 / sin5, 0.0)
     # a2b_ord4 edge weights (fv3core/grid/geometry.py:604-713): at a corner
     # point on a tile edge, the weight of the along-edge neighbour behind it,
     # d2 / (d1 + d2) with d1, d2 the distances to the midpoints of the A-grid

@@ -292,6 +292,17 @@ class CubedSphereCommunicator:
                                 n_halo: int = 3):
         return _GroupUpdater(HaloUpdater(self._gi, kind, n_halo))
 
+    def vector_halo_update(self, x_quantity, y_quantity, n_points: int):
+        """Blocking halo update of a D-grid wind pair (u on y-faces, v on
+        x-faces), ``n_points`` deep."""
+        key = ("vector_d", int(n_points))
+        if not hasattr(self, "_vector_updaters"):
+            self._vector_updaters = {}
+        upd = self._vector_updaters.get(key)
+        if upd is None:
+            upd = self._vector_updaters[key] = HaloUpdater(self._gi, "vector_d", int(n_points))
+        upd.update(*_GroupUpdater._storages([x_quantity, y_quantity]))
+
     def synchronize_vector_interfaces(self, x_quantity, y_quantity):
         """Make the cube-edge values of a D-grid wind pair agree between tiles
         (see :func:`build_sync_map`)."""

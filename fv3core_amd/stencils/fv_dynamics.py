@@ -6,13 +6,13 @@ then k_split times the copy delp -> dp1, AcousticDynamics, TracerAdvection
 and LagrangianToEulerian, and after the last remap set_omega, the omega
 halo update and HyperdiffusionDamping of omga.
 
-The wrap-up the reference runs at the end of the step (fv_dynamics.py:
-136-169) is not built: AdjustNegativeTracerMixingRatio (neg_adj3), which
-corrects negative qvapor, qliquid, qrain, qsnow, qice, qgraupel and qcld
-(moving water between species and adjusting pt), and CubedToLatLon, which
-recomputes ua and va from the D-grid winds.  A caller must say so by
-constructing DynamicalCore with ``skip_wrapup=True``; the step then leaves
-those nine fields as the remap leaves them.
+The step ends with the reference's wrap-up (fv_dynamics.py:136-169):
+AdjustNegativeTracerMixingRatio (neg_adj3) corrects negative qvapor,
+qliquid, qrain, qsnow, qice, qgraupel and qcld, moving water between species
+and adjusting pt, and CubedToLatLon recomputes ua and va from the D-grid
+winds (after a vector halo update of u and v for the 4th-order form).
+``skip_wrapup=True`` (no counterpart in the reference) leaves those nine
+fields as the last remap leaves them.
 """
 from __future__ import annotations
 
@@ -28,9 +28,11 @@ from ..constants import KAPPA, ZVIR
 from ..utils import gt4py_utils as utils
 from ..utils.grid import DampingCoefficients, GridData
 from ..utils.stencil import StencilFactory
+from .c2l_ord import CubedToLatLon
 from .del2cubed import HyperdiffusionDamping
 from .dyn_core import AcousticDynamics
 from .fvtp2d import FiniteVolumeTransport
+from .neg_adj3 import AdjustNegativeTracerMixingRatio
 from .remapping import LagrangianToEulerian
 from .tracer_2d_1l import TracerAdvection
 
@@ -86,16 +88,13 @@ class DynamicalCore:
                  use_graph: bool = False, skip_wrapup: bool = False):
         """``use_graph`` (no counterpart in the reference): capture a whole
         step (every k_split iteration of the acoustic substeps, tracer
-        transport and remap) in one HIP graph and replay it on later steps
-        with the same storages and scalars; the first step for a new set runs
-        eagerly, the second captures and replays."""
-        if not skip_wrapup:
-            raise NotImplementedError(
-                "the step's wrap-up (fv_dynamics.py:136-169) is not implemented: "
-                "AdjustNegativeTracerMixingRatio would correct negative qvapor, qliquid, qrain, "
-                "qsnow, qice, qgraupel and qcld and adjust pt, CubedToLatLon would recompute ua "
-                "and va; pass skip_wrapup=True to step without it")
+        transport and remap, and the wrap-up) in one HIP graph and replay it
+        on later steps with the same storages and scalars; the first step for
+        a new set runs eagerly, the second captures and replays.
+        ``skip_wrapup`` (no counterpart in the reference): end the step after
+        the last remap, without neg_adj3 and CubedToLatLon."""
         gi = stencil_factory.grid_indexing
+        self._skip_wrapup = bool(skip_wrapup)
         self._use_graph = bool(use_graph)
         self._graphs = {}
         self._seen_keys = set()
@@ -135,6 +134,11 @@ class DynamicalCore:
             stencil_factory, config.remapping, getattr(grid_data, "area_64", None), NQ,
             self._pfull)
         self._omega_halo_updater = comm.get_scalar_halo_updater()
+        if not self._skip_wrapup:
+            self._cubed_to_latlon = CubedToLatLon(stencil_factory, grid_data,
+                                                  order=config.c2l_ord)
+            self._adjust_tracer_mixing_ratio = AdjustNegativeTracerMixingRatio(
+                stencil_factory, config.check_negative, config.hydrostatic)
         self._temporaries = None
 
     # ------------------------------------------------------------ helpers
@@ -268,3 +272,9 @@ class DynamicalCore:
                     if config.nf_omega > 0:
                         self._omega_halo_updater.update([state.omga])
                         self._hyperdiffusion(state.omga, 0.18 * self._da_min)
+        if not self._skip_wrapup:
+            # wrapup (fv_dynamics.py:136-169)
+            self._adjust_tracer_mixing_ratio(
+                state.qvapor, state.qliquid, state.qrain, state.qsnow, state.qice,
+                state.qgraupel, state.qcld, state.pt, state.delp, state.delz, state.peln)
+            self._cubed_to_latlon(state.u, state.v, state.ua, state.va, self.comm)

@@ -582,6 +582,32 @@ int32_t fv3_fv_setup_f64(const fv3_domain* d, const double* const* q6, const dou
 int32_t fv3_set_omega_f64(const fv3_domain* d, const double* delp, const double* delz,
                           const double* w, double* omga, void* stream);
 
+/* ---- the step's wrap-up ----------------------------------------------- */
+
+/*
+ * AdjustNegativeTracerMixingRatio.__call__ (fv3core/stencils/neg_adj3.py:
+ * 335-372), nonhydrostatic, check_negative off: fix_neg_water, fillq on
+ * graupel then rain, fix_water_vapor_down and fix_neg_cloud on the compute
+ * box, layers [0, d->nk) (d->nk >= 3).  Updates the seven water species
+ * and pt in place; delp is read only.
+ */
+int32_t fv3_neg_adj3_f64(const fv3_domain* d, double* qvapor, double* qliquid, double* qrain,
+                         double* qsnow, double* qice, double* qgraupel, double* qcld, double* pt,
+                         const double* delp, void* stream);
+
+/*
+ * CubedToLatLon's stencil (fv3core/stencils/c2l_ord.py:17-72, called at
+ * :118-141): D-grid winds u, v (halos already filled) to A-grid
+ * latitude-longitude winds ua, va with the IJ transform a11..a22
+ * (fv3core/grid/geometry.py:596-601).  order 4: ord4_transform on the
+ * compute box; order 2: c2l_ord2 on the compute box grown by one.
+ * Layers [0, d->nk).
+ */
+int32_t fv3_cubed_to_latlon_f64(const fv3_domain* d, int32_t order, const double* u,
+                                const double* v, const double* dx, const double* dy,
+                                const double* a11, const double* a12, const double* a21,
+                                const double* a22, double* ua, double* va, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

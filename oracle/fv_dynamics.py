@@ -1,5 +1,6 @@
 """CPU oracle for DynamicalCore.step_dynamics (fv3core/stencils/fv_dynamics.py:
-193-557) without its wrap-up (neg_adj3, CubedToLatLon).
+193-557), with its wrap-up (neg_adj3, CubedToLatLon; fv_dynamics.py:136-169)
+when asked.
 
 Test infrastructure only: imported by tests, the smoke check and the CPU
 baseline of the benchmark, never by the product path.  Composes the oracle's
@@ -15,6 +16,7 @@ from . import remapping as orm
 from .dyncore import acoustic_dynamics, del2cubed
 from .fvtp2d import Tile
 from .tracer import tracer_advection
+from .wrapup import c2l_ord2, c2l_ord4, neg_adj3
 
 RDGAS = orm.RDGAS
 RVGAS = orm.RVGAS
@@ -48,7 +50,7 @@ def fv_setup_pt_adjust(S, t, nz):
 
 
 def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, nz,
-                edges=True):
+                edges=True, wrapup=False):
     """One step_dynamics on a batch of tiles, in place on ``S``.
 
     ``edges``: the tile-edge flags (True for the cube tiles, False for a
@@ -60,7 +62,8 @@ def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, 
     (the step), ptop, ks, k_split, n_split, hord_tr, kord_*, fill, z_tracer,
     nf_omega, p_ref.  ``halo(kind, names)`` refreshes state halos in place;
     ``tracer_halo(list)`` returns refreshed copies; ``sync()`` joins the
-    D-grid edge winds.
+    D-grid edge winds.  ``wrapup``: end with neg_adj3 and CubedToLatLon
+    (order ``cfg["c2l_ord"]``, default 4; the metrics hold a11..a22).
     """
     n = S["delp"].shape[1] - 7
     ntiles = S["delp"].shape[0]
@@ -108,3 +111,22 @@ def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, 
                     for ti in range(ntiles):
                         om = S["omga"][ti][..., :nz]
                         del2cubed(om, 0.18 * scalars["da_min"], metrics[ti], t, cfg["nf_omega"])
+    if wrapup:
+        C = (slice(t.isc, t.iec + 1), slice(t.jsc, t.jec + 1), slice(0, nz))
+        names = ("qvapor", "qliquid", "qrain", "qsnow", "qice", "qgraupel", "qcld", "pt")
+        for ti in range(ntiles):
+            o = neg_adj3(*(S[k][ti][C] for k in names), S["delp"][ti][C])
+            for k in names:
+                S[k][ti][C] = o[k]
+        order = cfg.get("c2l_ord", 4)
+        if order == 4:
+            halo("vector_d", ["u", "v"])
+        c2l = c2l_ord4 if order == 4 else c2l_ord2
+        g = 0 if order == 4 else 1
+        box = (slice(t.isc - g, t.iec + 1 + g), slice(t.jsc - g, t.jec + 1 + g), slice(0, nz))
+        for ti in range(ntiles):
+            m = metrics[ti]
+            ua, va = c2l(S["u"][ti][..., :nz], S["v"][ti][..., :nz], m["dx"], m["dy"], m["a11"],
+                         m["a12"], m["a21"], m["a22"], t.isc, t.iec, t.jsc, t.jec)
+            S["ua"][ti][box] = ua[box]
+            S["va"][ti][box] = va[box]

@@ -136,7 +136,7 @@ def _tracer_halo_fn():
     return lambda qs: [apply_map_numpy([q], mp)[0] for q in qs]
 
 
-def _oracle_step(init, cfg, bdt, ptop, seed):
+def _oracle_step(init, cfg, bdt, ptop, seed, wrapup=False):
     """The oracle driver on a copy of the initial state; with a seed, pt, delp,
     u, v and the tracers are first moved by one ulp with random signs."""
     from fv3core_amd.stencils.d_sw import get_column_namelist
@@ -168,17 +168,18 @@ def _oracle_step(init, cfg, bdt, ptop, seed):
                 p_fac=cfg.p_fac, rf_cutoff=cfg.rf_cutoff, tau=cfg.tau, delt_max=cfg.delt_max,
                 kord_tm=cfg.kord_tm, kord_tr=cfg.kord_tr, kord_wz=cfg.kord_wz,
                 kord_mt=cfg.kord_mt, fill=cfg.fill, z_tracer=cfg.z_tracer,
-                nf_omega=cfg.nf_omega, p_ref=cfg.p_ref)
+                nf_omega=cfg.nf_omega, p_ref=cfg.p_ref, c2l_ord=cfg.c2l_ord)
     metrics = [{k: v[ti] for k, v in m.items() if isinstance(v, np.ndarray) and v.ndim == 3}
                for ti in range(6)]
     ofd.fv_dynamics(S, ocfg, metrics, dict(da_min=m["da_min"], da_min_c=m["da_min_c"]), ak,
-                    bk, halo, _tracer_halo_fn(), halo.sync, col, NZ)
+                    bk, halo, _tracer_halo_fn(), halo.sync, col, NZ, wrapup=wrapup)
     return S
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("k_split,kord", [(1, 9), (2, 9), (1, 10), (2, 10)])
-def test_gpu_dynamical_core_step_matches_oracle(k_split, kord):
+@pytest.mark.parametrize("k_split,kord,wrapup", [(1, 9, False), (2, 9, False), (1, 10, False),
+                                                 (2, 10, False), (1, 9, True), (2, 9, True)])
+def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup):
     """kord 9: every field within 1e-9 (w and omga 1e-7).  kord 10: its
     extremum tests (|x0| > |a2 - a3|, remap_profile.py:271-277) switch the
     reconstruction discontinuously, so an input that sits within last-bit
@@ -188,7 +189,9 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord):
     levels of a column but keep its integral: for kord 10 the share of points
     that differ by more than 1e-9 may be at most 3x that share between the
     oracle and the oracle on inputs moved by one ulp (plus 1%), and the column
-    integrals of delz and of every tracer's mass must agree to 1e-9."""
+    integrals of delz and of every tracer's mass must agree to 1e-9.  With
+    the wrap-up (neg_adj3 and CubedToLatLon), qcld, ua and va are compared
+    too."""
     from fv3core_amd.halo import CubedSphereCommunicator
     from fv3core_amd.stencils.fv_dynamics import DynamicalCore
     from fv3core_amd.utils import gt4py_utils as utils
@@ -219,7 +222,7 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord):
     damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
     comm = CubedSphereCommunicator(gi)
     phis = dev3(init["phis"])
-    core = DynamicalCore(comm, grid, factory, damp, cfg, ak, bk, phis, skip_wrapup=True)
+    core = DynamicalCore(comm, grid, factory, damp, cfg, ak, bk, phis, skip_wrapup=not wrapup)
     state = {k: dev4(v) for k, v in st0.items() if k not in ("cappa",)}
     state.update({k: dev4(v) for k, v in tr0.items()})
     state["phis"] = phis
@@ -228,8 +231,8 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord):
     core.step_dynamics(state, 0.0, False, bdt, ptop, cfg.n_split, 0)
     torch.cuda.synchronize()
 
-    S = _oracle_step(init, cfg, bdt, ptop, None)
-    P = _oracle_step(init, cfg, bdt, ptop, 1) if kord == 10 else None
+    S = _oracle_step(init, cfg, bdt, ptop, None, wrapup)
+    P = _oracle_step(init, cfg, bdt, ptop, 1, wrapup) if kord == 10 else None
 
     def share(a, b, tol):
         scale = np.maximum(np.abs(a) + np.abs(b), 2e-6 * np.abs(b).max())
@@ -259,6 +262,9 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord):
     check("ps", (slice(None), slice(3, N + 3), slice(3, N + 3)), 1e-12)
     for name in ("mfxd", "mfyd", "cxd", "cyd"):
         check(name, c)
+    if wrapup:
+        for name in ("qcld", "ua", "va"):
+            check(name, c)
     if kord == 10:
         dp_got = state["delp"].cpu().numpy()[c]
 
