@@ -543,12 +543,14 @@ def _ensemble_parts(cfg, mixed):
     from fv3core_amd.ensemble import ensemble_acoustic
     from fv3core_amd.workloads import cube_state
 
-    plan = [(48, 32)] + [(n, 1) for n in (24, 64, 96, 128)]
+    # 32 perturbed C48 members, then the uneven members built the same way
+    # without perturbation
+    plan = [(48, 32, 0.5, 1.0)] + [(n, 1, 0.0, 0.0) for n in (24, 64, 96, 128)]
     parts, cells = [], 0
-    for n, members in plan:
+    for n, members, pt_sd, uv_sd in plan:
         init = cube_state(n, cfg.npz, seed=0)
-        parts.append(ensemble_acoustic(init, members, cfg, seed=1000 * n,
-                                       mixed_precision=mixed))
+        parts.append(ensemble_acoustic(init, members, cfg, seed=1000 * n, pt_sd=pt_sd,
+                                       uv_sd=uv_sd, mixed_precision=mixed))
         cells += members * 6 * n * n * cfg.npz * cfg.n_split
         del init
     return parts, cells
@@ -591,19 +593,20 @@ def _time_ensemble(parts, steps, warmup, roofline_kernel, graph):
     return wall, graph_wall, cnt, kern_ms, kern_bytes, finite
 
 
-def _ensemble_mixed_errors(cfg):
-    """Errors of the mixed form against the FP64 form after one call on the
-    uneven C96 member (same initial state, same perturbation)."""
+def _ensemble_mixed_errors(cfg, n=96, pt_sd=0.0, uv_sd=0.0, seed=96000):
+    """Errors of the mixed form against the FP64 form after one call on one
+    member at C``n`` (the same initial state and perturbation for both)."""
     import torch
 
     from fv3core_amd.ensemble import ensemble_acoustic, member_slice
     from fv3core_amd.workloads import cube_state
 
-    init = cube_state(96, cfg.npz, seed=0)
+    init = cube_state(n, cfg.npz, seed=0)
     built = {}
 
     def make(mixed):
-        acoustic, state = ensemble_acoustic(init, 1, cfg, seed=96000, mixed_precision=mixed)
+        acoustic, state = ensemble_acoustic(init, 1, cfg, seed=seed, pt_sd=pt_sd, uv_sd=uv_sd,
+                                            mixed_precision=mixed)
         built[mixed] = state
         return acoustic
 
@@ -612,7 +615,7 @@ def _ensemble_mixed_errors(cfg):
     plain(sp)
     mixed(sm)
     torch.cuda.synchronize()
-    n, nz = 96, cfg.npz
+    nz = cfg.npz
     c = (slice(None), slice(3, n + 3), slice(3, n + 3))
 
     def rel(a, b):
@@ -650,7 +653,10 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
 
     nz, n_split = 79, 6
     cfg = DynamicalCoreConfig(npz=nz, n_split=n_split, k_split=1, nord=3, dt_atmos=225)
+    # the gate is read on the unperturbed C96 member; a perturbed C48 member
+    # (grid-scale noise, the worst case for the limiters) is reported beside it
     errs = _ensemble_mixed_errors(cfg)
+    errs_pert = _ensemble_mixed_errors(cfg, n=48, pt_sd=0.5, uv_sd=1.0, seed=48000)
     results = {}
     cells = 0
     for form in ("fp64", "mixed"):
@@ -676,6 +682,8 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
         "mixed_over_fp64": wall64 / used,
         "mixed_vs_fp64": errs,
         "mixed_gate_ok": max(errs[k] for k in ("ps", "pt", "delp")) <= 1e-5,
+        "mixed_vs_fp64_perturbed_c48": errs_pert,
+        "mixed_gate_ok_perturbed_c48": max(errs_pert[k] for k in ("ps", "pt", "delp")) <= 1e-5,
         "step_roofline": {
             "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBPS,
             "bytes_per_cell_substep": STEP_BYTES_PER_CELL_FP64,

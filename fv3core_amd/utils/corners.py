@@ -21,10 +21,16 @@ class CopyCornersXY:
         self._grid_indexing = stencil_factory.grid_indexing
         self._dims = tuple(dims)
         self._y_field = y_field
+        # the levels the dims name (corners.py:84-86): nz for Z_DIM, nz + 1
+        # for Z_INTERFACE_DIM; levels beyond them are left alone
+        _, domain = self._grid_indexing.get_origin_domain(dims=self._dims)
+        self._nk = int(domain[2]) if len(domain) > 2 else 1
 
     def __call__(self, field):
         _lib.require_device(field, self._y_field)
-        nk = field.shape[-1]
+        nk = self._nk
+        if field.shape[-1] < nk:
+            raise ValueError(f"the dims name {nk} levels but the field has {field.shape[-1]}")
         d = _lib.make_domain(self._grid_indexing, field, nk, others3=(self._y_field,))
         _lib.check(
             _lib.fn("fv3_copy_corners_xy", field.dtype)(

@@ -238,12 +238,12 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup):
         scale = np.maximum(np.abs(a) + np.abs(b), 2e-6 * np.abs(b).max())
         return float((2.0 * np.abs(a - b) / scale > tol).mean())
 
-    def check(name, box, tol=None):
+    def check(name, box, tol=None, scale_floor=1e-6):
         got = state[name].cpu().numpy()[box]
         want = S[name][box]
         tol = tol or TOL.get(name, 1e-9)
         if kord == 9:
-            assert_close(got, want, tol, near_zero=1e-14, what=name, scale_floor=1e-6)
+            assert_close(got, want, tol, near_zero=1e-14, what=name, scale_floor=scale_floor)
             return
         frac = share(got, want, tol)
         ref = share(P[name][box], want, tol)
@@ -263,8 +263,13 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup):
     for name in ("mfxd", "mfyd", "cxd", "cyd"):
         check(name, c)
     if wrapup:
-        for name in ("qcld", "ua", "va"):
-            check(name, c)
+        check("qcld", c)
+        # ua and va are sums of u and v terms with O(1) weights: where the
+        # terms cancel, their absolute error is that of u and v (1e-9 of the
+        # wind scale), so points below 1e-4 of the field's scale are held to
+        # that absolute bound
+        for name in ("ua", "va"):
+            check(name, c, scale_floor=1e-4)
     if kord == 10:
         dp_got = state["delp"].cpu().numpy()[c]
 

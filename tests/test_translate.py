@@ -175,7 +175,12 @@ def test_gpu_translate_riem_solver3(tmp_path, last_call):
     namelist = SimpleNamespace(p_fac=0.05, a_imp=1.0, use_logp=False, beta=0.0)
     serializer = NpzSerializer(tmp_path)
     testobj = TRANSLATE_CLASSES["Riem_Solver3"](g, namelist)
-    testobj.max_error = 1e-10  # the reference's threshold for GPU backends
+    # The reference allows 5e-6 on GPU backends
+    # (tests/savepoint/translate/overrides/standard.yaml:46-52).  ppe and w
+    # are small residuals of the implicit solve whose last bits follow
+    # exp/log, which differ between ROCm and numpy: on these inputs one ulp
+    # on pt moves ppe by 1.3e-9 and w by 2.7e-9 in the oracle itself.
+    testobj.max_error = 5e-9
     inputs = testobj.collect_input_data(serializer, "Riem_Solver3-In")
     output = testobj.compute(inputs)
     failures = {k: v for k, v in testobj.check(serializer, "Riem_Solver3-Out", output).items()
