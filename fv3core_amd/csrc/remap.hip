@@ -6,6 +6,8 @@
 #include "remap_profile.hpp"
 #include "timing.hpp"
 
+#include <mutex>
+
 namespace fv3 {
 
 constexpr int64_t kFusedRemapWaves = 1024;
@@ -55,9 +57,72 @@ static int map_single_launch(const fv3_domain* dom, int kord, int iv, int i1, in
   return FV3_OK;
 }
 
+// ---- scratch of the entry point that allocates nothing
+// The six column slabs of map_single_kernel live in one file-scope device
+// array; every GPU the library runs on holds its own copy until the process
+// ends.  Host code never passes the array's name to a kernel (on the host it
+// is only a stand-in): it uses the device address hipGetSymbolAddress gives
+// for the current device, kept per device.
+constexpr size_t kNoAllocScratchBytes = size_t(96) << 20;
+constexpr int kRemapSlabs = 6;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];
+
+static int scratch_on_current_device(void** out, const char* who) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  FV3_REQUIRE(e == hipSuccess, FV3_E_HIP, "%s: hipGetDevice failed: %s", who,
+              hipGetErrorString(e));
+  FV3_REQUIRE(dev >= 0 && dev < kMaxDevices, FV3_E_HIP, "%s: device %d out of range", who, dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    FV3_REQUIRE(e == hipSuccess && p != nullptr, FV3_E_HIP,
+                "%s: hipGetSymbolAddress(scratch) failed on device %d: %s", who, dev,
+                hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FV3_OK;
+}
+
+static int map_single_noalloc(const fv3_domain* dom, int kord, int iv, int i1, int i2, int j1,
+                              int j2, double* q1, const double* pe1, const double* pe2,
+                              const double* qs, double qmin, void* stream) {
+  const char* who = "fv3_map_single_noalloc_f64";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(dom->st >= 0, FV3_E_ARG, "%s: negative tile stride", who);
+  const int64_t tiles_before = dom->ntiles > 0 ? dom->ntiles - 1 : 0;
+  // slab length in doubles, rounded to 256 bytes so every slab stays aligned
+  const int64_t slab = (tiles_before * dom->st + dom->sk * (int64_t)(dom->nk + 1) + 31) / 32 * 32;
+  const int64_t max_slab = (int64_t)(kNoAllocScratchBytes / sizeof(double) / kRemapSlabs);
+  FV3_REQUIRE(slab <= max_slab, FV3_E_ARG,
+              "%s: the columns need %lld bytes of scratch (6 slabs of %lld doubles), above the "
+              "%zu-byte maximum",
+              who, (long long)(slab * kRemapSlabs * (int64_t)sizeof(double)), (long long)slab,
+              kNoAllocScratchBytes);
+  void* p = nullptr;
+  rc = scratch_on_current_device(&p, who);
+  if (rc) return rc;
+  return map_single_launch<double>(dom, kord, iv, i1, i2, j1, j2, q1, pe1, pe2, qs, qmin,
+                                   static_cast<double*>(p), slab, stream);
+}
+
 }  // namespace fv3
 
 extern "C" {
+
+int32_t fv3_map_single_noalloc_f64(const fv3_domain* d, int32_t kord, int32_t mode, int32_t i1,
+                                   int32_t i2, int32_t j1, int32_t j2, double* q1,
+                                   const double* pe1, const double* pe2, const double* qs,
+                                   double qmin, void* stream) {
+  return fv3::map_single_noalloc(d, kord, mode, i1, i2, j1, j2, q1, pe1, pe2, qs, qmin, stream);
+}
 
 int32_t fv3_map_single_f64(const fv3_domain* d, int32_t kord, int32_t mode, int32_t i1,
                            int32_t i2, int32_t j1, int32_t j2, double* q1, const double* pe1,

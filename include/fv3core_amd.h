@@ -219,6 +219,24 @@ int32_t fv3_map_single_f32(const fv3_domain* d, int32_t kord, int32_t mode, int3
                            int64_t work_slab, void* stream);
 
 /*
+ * fv3_map_single_f64 without the work argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The six column slabs live in a
+ * file-scope device array of 96 MiB held by every GPU the library runs on
+ * until the process ends; a call needing more, i.e. with
+ * 6 * 8 * ((ntiles-1)*st + (km+1)*sk) bytes above 96 MiB (one C128 tile,
+ * one C96 tile or six C48 tiles at 79 levels fit), fails with FV3_E_ARG
+ * before any launch.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other host threads) race.  The
+ * kernel writes every scratch entry it reads, so nothing needs clearing.
+ */
+int32_t fv3_map_single_noalloc_f64(const fv3_domain* d, int32_t kord, int32_t mode, int32_t i1,
+                                   int32_t i2, int32_t j1, int32_t j2, double* q1,
+                                   const double* pe1, const double* pe2, const double* qs,
+                                   double qmin, void* stream);
+
+/*
  * Tangent-linear model and adjoint of fv3_map_single_f64 (map_scalar), kord 9,
  * with respect to the remapped field, at the branch decisions of the forward;
  * pe1, pe2, qs and qmin are held fixed.  No counterpart in the reference,
