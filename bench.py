@@ -53,6 +53,13 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+MIXED_ERROR_FIELDS = ("delp", "pt", "delz", "w", "u", "v", "zh", "pe", "pk", "peln", "pkz", "ua",
+                      "va", "uc", "vc", "omga")
+MIXED_FORM = ("FP32 storages and arithmetic for the bulk fields (u, v, w, delp, pt, q_con, "
+              "cappa, C-grid winds, fluxes and accumulators) in c_sw, d_sw (fv_tp_2d, del-n "
+              "and divergence damping, a2b_ord4), Rayleigh damping, del2cubed and their halo "
+              "exchange; FP64 heights, pressures (pe, peln, pk, pk3, pkc, pkz, zh, gz, delz), "
+              "Riemann solvers, height updates and pressure gradients")
 
 
 def _parse():
@@ -71,7 +78,7 @@ def _parse():
     p.add_argument("--graph", action="store_true",
                    help="dyncore: also time the step replayed from a captured HIP graph")
     p.add_argument("--mixed", action="store_true",
-                   help="dyncore: FP32 arithmetic in d_sw's transports; reports the error of "
+                   help="dyncore: the mixed-precision form (FP32 bulk fields); reports the error of "
                         "every field against the FP64 form")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -236,23 +243,40 @@ def _mixed_errors(make, state_of, n, nz):
     surface."""
     import torch
 
+    from fv3core_amd.mixed import mixed_state
+
     plain, mixed = make(False), make(True)
-    sp, sm = state_of(), state_of()
+    sp, sm = state_of(), mixed_state(state_of())
     plain(sp)
     mixed(sm)
     torch.cuda.synchronize()
     c = (slice(None), slice(3, n + 3), slice(3, n + 3))
 
     def rel(a, b):
+        a, b = a.double(), b.double()
         scale = float(torch.maximum(a.abs().max(), b.abs().max()))
         den = torch.clamp(a.abs() + b.abs(), min=1e-6 * scale + 1e-300)
         return float((2.0 * (a - b).abs() / den).max())
 
     errs = {k: rel(getattr(sm, k)[c + (slice(0, nz),)], getattr(sp, k)[c + (slice(0, nz),)])
-            for k in ("delp", "pt", "delz", "w", "u", "v", "zh", "pe", "pk", "peln", "pkz", "ua",
-                      "va", "uc", "vc", "omga")}
+            for k in MIXED_ERROR_FIELDS}
     errs["ps"] = rel(sm.pe[c + (nz,)], sp.pe[c + (nz,)])
+    errs["scaled"] = _scaled_errors(sm, sp, n, nz)
     return errs
+
+
+def _scaled_errors(sm, sp, n, nz):
+    """max|a - b| / max|b| per field: the error against the field's scale,
+    which stays meaningful for fields that pass through zero (w, winds)."""
+    import torch
+
+    c = (slice(None), slice(3, n + 3), slice(3, n + 3))
+    out = {}
+    for k in MIXED_ERROR_FIELDS:
+        a = getattr(sm, k)[c + (slice(0, nz),)].double()
+        b = getattr(sp, k)[c + (slice(0, nz),)].double()
+        out[k] = float((a - b).abs().max() / torch.clamp(b.abs().max(), min=1e-300))
+    return out
 
 
 def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = False,
@@ -304,6 +328,10 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
     mixed_errs = _mixed_errors(make, state_of, n, nz) if mixed else None
     acoustic = make(mixed)
     state = state_of()
+    if mixed:
+        from fv3core_amd.mixed import mixed_state
+
+        state = mixed_state(state)
     lib = _lib.lib()
     for _ in range(warmup):
         acoustic(state)
@@ -341,8 +369,7 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
         "config": {
             "workload": f"AcousticDynamics (dyn_core) on 6 cube tiles x {n}x{n}x{nz}, "
             f"n_split={n_split}, dt=225 s, nonhydrostatic, jet + 1 K pt bump initial state" +
-            (", mixed precision: FP32 arithmetic in d_sw's fv_tp_2d transports, FP64 storages "
-             "and every other kernel FP64" if mixed else ""),
+            (", mixed precision: " + MIXED_FORM if mixed else ""),
             "tiles": 6, "n": n, "nz": nz, "n_split": n_split,
             "cells_per_step": cells, "state_finite_after_timed_steps": finite,
         },
@@ -598,7 +625,7 @@ def _ensemble_mixed_errors(cfg, n=96, pt_sd=0.0, uv_sd=0.0, seed=96000):
     member at C``n`` (the same initial state and perturbation for both)."""
     import torch
 
-    from fv3core_amd.ensemble import ensemble_acoustic, member_slice
+    from fv3core_amd.ensemble import ensemble_acoustic
     from fv3core_amd.workloads import cube_state
 
     init = cube_state(n, cfg.npz, seed=0)
@@ -611,7 +638,7 @@ def _ensemble_mixed_errors(cfg, n=96, pt_sd=0.0, uv_sd=0.0, seed=96000):
         return acoustic
 
     plain, mixed = make(False), make(True)
-    sp, sm = built[False], member_slice(built[False], 0)
+    sp, sm = built[False], built[True]  # one initial state, FP64 and mixed storages
     plain(sp)
     mixed(sm)
     torch.cuda.synchronize()
@@ -619,14 +646,15 @@ def _ensemble_mixed_errors(cfg, n=96, pt_sd=0.0, uv_sd=0.0, seed=96000):
     c = (slice(None), slice(3, n + 3), slice(3, n + 3))
 
     def rel(a, b):
+        a, b = a.double(), b.double()
         scale = float(torch.maximum(a.abs().max(), b.abs().max()))
         den = torch.clamp(a.abs() + b.abs(), min=1e-6 * scale + 1e-300)
         return float((2.0 * (a - b).abs() / den).max())
 
     errs = {k: rel(getattr(sm, k)[c + (slice(0, nz),)], getattr(sp, k)[c + (slice(0, nz),)])
-            for k in ("delp", "pt", "delz", "w", "u", "v", "zh", "pe", "pk", "peln", "pkz", "ua",
-                      "va", "uc", "vc", "omga")}
+            for k in MIXED_ERROR_FIELDS}
     errs["ps"] = rel(sm.pe[c + (nz,)], sp.pe[c + (nz,)])
+    errs["scaled"] = _scaled_errors(sm, sp, n, nz)
     return errs
 
 
@@ -639,8 +667,7 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
     one AcousticDynamics call (n_split = 6, dt = 225 s) of every batch;
     cell-level updates per step = sum over members of 6 n^2 79 6.
 
-    Both forms are timed: plain FP64 and the mixed form (FP32 arithmetic in
-    d_sw's fv_tp_2d transports, FP64 storages, every other kernel FP64).
+    Both forms are timed: plain FP64 and the mixed form (MIXED_FORM).
     ``value`` is the mixed form's rate, with the FP64 rate, their ratio, the
     mixed form's per-field error against the FP64 form (one call on the C96
     member) and, unless ``adjoint`` is off, the adjoint/forward time ratio of
@@ -694,8 +721,7 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
         "config": {
             "workload": "batched ensemble: 32 members x 6 x C48 x 79 in one batch + uneven "
             "members C24/C64/C96/C128 (one cube each), AcousticDynamics n_split=6, dt=225 s, "
-            "perturbed jet states; mixed form: FP32 arithmetic in d_sw's fv_tp_2d transports, "
-            "FP64 storages, riem solvers, remap and every other kernel FP64" +
+            "perturbed jet states; mixed form: " + MIXED_FORM +
             (", each batch replayed as a HIP graph" if graph else ""),
             "members": 36, "tiles": 6 * 36, "nz": nz, "n_split": n_split,
             "cells_per_step": cells,

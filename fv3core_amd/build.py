@@ -76,9 +76,10 @@ def build(verbose: bool = True) -> pathlib.Path:
             f.result()
     out = HERE / LIB_NAME
     tmp = HERE / (LIB_NAME + ".tmp")
-    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={OFFLOAD_ARCH}"] + [
-        str(o) for o in objs
-    ] + ["-o", str(tmp)]
+    # --no-undefined: a template used in one file but instantiated in none
+    # fails the link here instead of at load time
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={OFFLOAD_ARCH}",
+           "-Wl,--no-undefined"] + [str(o) for o in objs] + ["-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

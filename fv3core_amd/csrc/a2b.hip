@@ -348,6 +348,9 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
 template int launch_a2b_ord4<double>(hipStream_t, const fv3_domain*, const Met<double>&, int, int,
                                      const double*, double*, double*, double*, int, double*,
                                      const char*);
+template int launch_a2b_ord4<float>(hipStream_t, const fv3_domain*, const Met<float>&, int, int,
+                                    const float*, float*, float*, float*, int, float*,
+                                    const char*);
 
 int check_a2b_metrics(const fv3_metrics* g, const char* who) {
   FV3_REQUIRE(g && g->dxa && g->dya && g->agrid1 && g->agrid2 && g->bgrid1 && g->bgrid2 &&

@@ -454,6 +454,57 @@ int check_csw_metrics(const fv3_metrics* g, int nord, const char* who) {
   return FV3_OK;
 }
 
+template <typename T>
+int run_c_sw(const fv3_domain* dom, const fv3_metrics* g, int32_t nord, T dt2, T* delp, T* pt,
+             const T* u, const T* v, T* w, T* uc, T* vc, T* ua, T* va, T* ut, T* vt, T* divgd,
+             T* omga, T* delpc, T* ptc, T* work, int64_t work_slab, void* stream) {
+  const char* who = "fv3_c_sw";
+  int rc = check_csw_domain(dom, who);
+  if (rc) return rc;
+  rc = check_csw_metrics(g, nord, who);
+  if (rc) return rc;
+  FV3_REQUIRE(delp && pt && u && v && w && uc && vc && ua && va && ut && vt && omga && delpc &&
+                  ptc && work && (nord <= 0 || divgd),
+              FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)dom->nk * dom->sk,
+              FV3_E_ARG, "%s: work_slab too small", who);
+  const Met<T> m = to_met<T>(g);
+  hipStream_t s = as_stream(stream);
+  T* utmp = work;
+  T* vtmp = work + work_slab;
+  T* fx = work + 2 * work_slab;
+  T* fx1 = work + 3 * work_slab;
+  T* fx2 = work + 4 * work_slab;
+  T* ke = work + 5 * work_slab;
+  T* vort = work + 6 * work_slab;
+  rc = run_d2a2c<T>(s, dom, m, 1, dt2, uc, vc, u, v, ua, va, ut, vt, utmp, vtmp, delp, pt, w,
+                    who);
+  if (rc) return rc;
+  Dom d = to_dom(dom);
+  launch_fluxx_div<T>(s, d, m, delp, pt, w, ut, fx, fx1, fx2, u, v, ua, va,
+                      nord > 0 ? divgd : nullptr);
+  FV3_HIP_CHECK_LAUNCH(who);
+  const Box edges{dom->isc - 3, dom->iec + 3, dom->jsc - 3, dom->jec + 3};
+  FillTable fy;
+  fy.n = 0;
+  FillFields<T> ff{};
+  ff.f[0] = delp;
+  ff.f[1] = pt;
+  ff.f[2] = w;
+  for (int f = 0; f < 3; ++f) append_fill(&fy, dom, kFill2Y, f, f, kUnitMult, edges);
+  launch_fill(s, d, fy, ff);
+  FV3_HIP_CHECK_LAUNCH(who);
+  launch_transport<T>(s, d, m, dt2, delp, pt, w, ut, vt, fx, fx1, fx2, ua, va, uc, vc, u, v,
+                      delpc, ptc, omga, ke);
+  FV3_HIP_CHECK_LAUNCH(who);
+  launch_vort<T>(s, d, m, uc, vc, vort);
+  FV3_HIP_CHECK_LAUNCH(who);
+  launch_update<T>(s, d, m, dt2, vort, ke, u, v, uc, vc);
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
 }  // namespace
 
 }  // namespace fv3
@@ -485,51 +536,16 @@ int32_t fv3_c_sw_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t nord, 
                      double* uc, double* vc, double* ua, double* va, double* ut, double* vt,
                      double* divgd, double* omga, double* delpc, double* ptc, double* work,
                      int64_t work_slab, void* stream) {
-  using namespace fv3;
-  const char* who = "fv3_c_sw";
-  int rc = check_csw_domain(dom, who);
-  if (rc) return rc;
-  rc = check_csw_metrics(g, nord, who);
-  if (rc) return rc;
-  FV3_REQUIRE(delp && pt && u && v && w && uc && vc && ua && va && ut && vt && omga && delpc &&
-                  ptc && work && (nord <= 0 || divgd),
-              FV3_E_ARG, "%s: NULL pointer", who);
-  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
-  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)dom->nk * dom->sk,
-              FV3_E_ARG, "%s: work_slab too small", who);
-  const Met<double> m = to_met<double>(g);
-  hipStream_t s = as_stream(stream);
-  double* utmp = work;
-  double* vtmp = work + work_slab;
-  double* fx = work + 2 * work_slab;
-  double* fx1 = work + 3 * work_slab;
-  double* fx2 = work + 4 * work_slab;
-  double* ke = work + 5 * work_slab;
-  double* vort = work + 6 * work_slab;
-  rc = run_d2a2c<double>(s, dom, m, 1, dt2, uc, vc, u, v, ua, va, ut, vt, utmp, vtmp, delp, pt,
-                         w, who);
-  if (rc) return rc;
-  Dom d = to_dom(dom);
-  launch_fluxx_div<double>(s, d, m, delp, pt, w, ut, fx, fx1, fx2, u, v, ua, va,
-                           nord > 0 ? divgd : nullptr);
-  FV3_HIP_CHECK_LAUNCH(who);
-  const Box edges{dom->isc - 3, dom->iec + 3, dom->jsc - 3, dom->jec + 3};
-  FillTable fy;
-  fy.n = 0;
-  FillFields<double> ff{};
-  ff.f[0] = delp;
-  ff.f[1] = pt;
-  ff.f[2] = w;
-  for (int f = 0; f < 3; ++f) append_fill(&fy, dom, kFill2Y, f, f, kUnitMult, edges);
-  launch_fill(s, d, fy, ff);
-  FV3_HIP_CHECK_LAUNCH(who);
-  launch_transport<double>(s, d, m, dt2, delp, pt, w, ut, vt, fx, fx1, fx2, ua, va, uc, vc, u, v,
-                           delpc, ptc, omga, ke);
-  FV3_HIP_CHECK_LAUNCH(who);
-  launch_vort<double>(s, d, m, uc, vc, vort);
-  FV3_HIP_CHECK_LAUNCH(who);
-  launch_update<double>(s, d, m, dt2, vort, ke, u, v, uc, vc);
-  FV3_HIP_CHECK_LAUNCH(who);
-  return FV3_OK;
+  return fv3::run_c_sw<double>(dom, g, nord, dt2, delp, pt, u, v, w, uc, vc, ua, va, ut, vt,
+                               divgd, omga, delpc, ptc, work, work_slab, stream);
+}
+
+int32_t fv3_c_sw_f32(const fv3_domain* dom, const fv3_metrics* g, int32_t nord, float dt2,
+                     float* delp, float* pt, const float* u, const float* v, float* w, float* uc,
+                     float* vc, float* ua, float* va, float* ut, float* vt, float* divgd,
+                     float* omga, float* delpc, float* ptc, float* work, int64_t work_slab,
+                     void* stream) {
+  return fv3::run_c_sw<float>(dom, g, nord, dt2, delp, pt, u, v, w, uc, vc, ua, va, ut, vt,
+                              divgd, omga, delpc, ptc, work, work_slab, stream);
 }
 }

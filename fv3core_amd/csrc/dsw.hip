@@ -21,6 +21,44 @@ __device__ __forceinline__ T contra(T v1, T v2, T cosa, T rsin2) {
   return (v1 - v2 * cosa) * rsin2;
 }
 
+// Typed views of fv3_dsw_params' columns and of the state structs (the
+// same member order for every element type).
+template <typename T>
+struct DswCols {
+  const T *damp_vt_deln, *damp_t_deln, *damp_w_nosg, *damp_v_nosg, *damp_w, *damp_vt, *ke_bg,
+      *d_con_col, *d2_divg;
+};
+
+template <typename T>
+DswCols<T> dsw_cols(const fv3_dsw_params* p) {
+  DswCols<T> c;
+  c.damp_vt_deln = static_cast<const T*>(p->damp_vt_deln);
+  c.damp_t_deln = static_cast<const T*>(p->damp_t_deln);
+  c.damp_w_nosg = static_cast<const T*>(p->damp_w_nosg);
+  c.damp_v_nosg = static_cast<const T*>(p->damp_v_nosg);
+  c.damp_w = static_cast<const T*>(p->damp_w);
+  c.damp_vt = static_cast<const T*>(p->damp_vt);
+  c.ke_bg = static_cast<const T*>(p->ke_bg);
+  c.d_con_col = static_cast<const T*>(p->d_con_col);
+  c.d2_divg = static_cast<const T*>(p->d2_divg);
+  return c;
+}
+
+template <typename T>
+struct DswState {
+  T *delpc, *delp, *ptc, *pt, *u, *v, *w, *uc, *vc, *ua, *va, *divgd;
+  T *mfx, *mfy, *cx, *cy, *crx, *cry, *xfx, *yfx, *q_con, *zh, *heat_source, *diss_est;
+};
+static_assert(sizeof(DswState<double>) == sizeof(fv3_dsw_state), "fv3_dsw_state layout");
+static_assert(sizeof(DswState<float>) == sizeof(fv3_dsw_state_f32), "fv3_dsw_state_f32 layout");
+
+template <typename T, typename S>
+DswState<T> dsw_state(const S* st) {
+  DswState<T> o;
+  memcpy(&o, st, sizeof(o));
+  return o;
+}
+
 // --------------------------------------------------------------------- fxadv
 // main_uc_contra + uc_contra_y_edge and main_vc_contra (fxadv.py:11-58, 61-71)
 // over the full domain.  Points the reference restores to their previous
@@ -401,19 +439,19 @@ void launch_ke(hipStream_t s, const Dom& d, const Met<T>& m, T dt, const T* uc, 
 // where the transport writes them) + heat_diss (compute domain).
 template <typename T>
 void launch_capacitor_heat(hipStream_t s, const Dom& d, const Met<T>& m, T dt, int nonhydro,
-                           const fv3_dsw_params* p, const fv3_dsw_state* st, const T* fx,
+                           const DswCols<T>& cols, const DswState<T>& st, const T* fx,
                            const T* fy, const T* fx2, const T* fy2, T* dw, T* heat_s) {
   const Box full{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
-  T* cx = st->cx;
-  T* cy = st->cy;
-  T* mfx = st->mfx;
-  T* mfy = st->mfy;
-  const T* crx = st->crx;
-  const T* cry = st->cry;
-  const T* w = st->w;
-  T* diss = st->diss_est;
-  const T* damp_w = p->damp_w;
-  const T* ke_bg = p->ke_bg;
+  T* cx = st.cx;
+  T* cy = st.cy;
+  T* mfx = st.mfx;
+  T* mfy = st.mfy;
+  const T* crx = st.crx;
+  const T* cry = st.cry;
+  const T* w = st.w;
+  T* diss = st.diss_est;
+  const T* damp_w = cols.damp_w;
+  const T* ke_bg = cols.ke_bg;
   launch_box(s, d, full, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
     const int64_t o = at3(d, i, j, k, t);
     cx[o] = cx[o] + crx[o];
@@ -676,8 +714,8 @@ void append_rotation_fill(FillTable* tab, const fv3_domain* dom, int kind, int f
 // Smagorinsky-like vorticity (from a2b_ord4 of wk) and the damping.
 template <typename T>
 int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv3_dsw_params* p,
-                 T dt, const T* wk, T* divg, T* vc, T* uc, T* delpc, T* vort, T* ke, T* a2b_qx,
-                 T* a2b_edges, const char* who) {
+                 const DswCols<T>& cols, T dt, const T* wk, T* divg, T* vc, T* uc, T* delpc,
+                 T* vort, T* ke, T* a2b_qx, T* a2b_edges, const char* who) {
   const int k0 = p->div_k0;
   fv3_domain hd = *dom;
   hd.nk = dom->nk - k0;
@@ -691,10 +729,17 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
   T* vo = vort + ko;
   T* keh = ke + ko;
   const T* wkh = wk + ko;
-  const T* d2 = p->d2_divg + k0;
+  const T* d2 = cols.d2_divg + k0;
   const int nord = p->div_nord;
   const bool corner = (has_edge(dom->i_start) || has_edge(dom->i_end)) &&
                       (has_edge(dom->j_start) || has_edge(dom->j_end));
+  // In FP32 the product dd8 * (nord-fold Laplacian of divg_d) over- and
+  // underflows on coarse grids (dd8 = (d4_bg da_min_c)^(nord+1) reaches 1e43
+  // at C12, the Laplacian 1e-41): there each pass also multiplies by da_min_c
+  // and the final coefficient divides that out, formed in FP64.  In FP64
+  // the factor is 1, which leaves every result bit for bit as it was.
+  const bool rescale = sizeof(T) < sizeof(double);
+  const T pass_scale = rescale ? (T)p->da_min_c : T(1);
   // copy_computeplus: delpc = divg_d on the corners (before the passes change divg_d)
   const Box comp{d.isc, d.iec + 1, d.jsc, d.jec + 1};
   launch_box(s, d, comp, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
@@ -755,7 +800,7 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
       } else {
         val = uch[o - sj] - uch[o] + vch[o - 1] - vch[o];
       }
-      dg[o] = val * m.rarea_c[at2(d, i, j, t)];
+      dg[o] = val * m.rarea_c[at2(d, i, j, t)] * pass_scale;
     });
     FV3_HIP_CHECK_LAUNCH(who);
   }
@@ -766,7 +811,8 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
                                 nullptr, who);
     if (rc) return rc;
   }
-  const T adt = fabs(dt), da_min_c = (T)p->da_min_c, dd8 = (T)p->dd8;
+  const T adt = fabs(dt), da_min_c = (T)p->da_min_c;
+  const T dd8 = rescale ? (T)(p->dd8 / pow(p->da_min_c, (double)nord)) : (T)p->dd8;
   const int zero_vort = dddmp < T(1e-5);
   launch_box(s, d, comp, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
     const int64_t o = at3(d, i, j, k, t);
@@ -787,11 +833,43 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
   return FV3_OK;
 }
 
-template <int ORD>
-void launch_ke_ord(hipStream_t s, const Dom& d, const Met<double>& m, double dt, const double* uc,
-                   const double* vc, const double* u, const double* v, const double* ucc,
-                   const double* vcc, double* ke) {
-  launch_ke<double, ORD>(s, d, m, dt, uc, vc, u, v, ucc, vcc, ke);
+template <typename T, int ORD>
+void launch_ke_ord(hipStream_t s, const Dom& d, const Met<T>& m, T dt, const T* uc, const T* vc,
+                   const T* u, const T* v, const T* ucc, const T* vcc, T* ke) {
+  launch_ke<T, ORD>(s, d, m, dt, uc, vc, u, v, ucc, vcc, ke);
+}
+
+// The transports of d_sw for each storage type: fv_tp_2d with its corner
+// fills, and the del-n damping of the fluxes.
+inline int dsw_transport(const fv3_domain* d, int hord, double* q, double* qy, const double* crx,
+                         const double* cry, const double* xfx, const double* yfx,
+                         const double* xmf, const double* ymf, const double* area,
+                         const double* dxa, const double* dya, double* fx, double* fy,
+                         bool fp32_arithmetic, void* stream) {
+  return fvtp2d_corner_fill_f64(d, hord, q, qy, crx, cry, xfx, yfx, xmf, ymf, area, dxa, dya, fx,
+                                fy, fp32_arithmetic, stream);
+}
+
+inline int dsw_transport(const fv3_domain* d, int hord, float* q, float* qy, const float* crx,
+                         const float* cry, const float* xfx, const float* yfx, const float* xmf,
+                         const float* ymf, const float* area, const float* dxa, const float* dya,
+                         float* fx, float* fy, bool, void* stream) {
+  return fvtp2d_corner_fill_f32(d, hord, q, qy, crx, cry, xfx, yfx, xmf, ymf, area, dxa, dya, fx,
+                                fy, stream);
+}
+
+inline int dsw_delnflux(const fv3_domain* d, int mode, const int* nord4, const double* q,
+                        const double* mass, const double* damp, const double* del6_u,
+                        const double* del6_v, const double* rarea, double* fx, double* fy,
+                        void* stream) {
+  return fv3_delnflux_f64(d, mode, nord4, q, mass, damp, del6_u, del6_v, rarea, fx, fy, stream);
+}
+
+inline int dsw_delnflux(const fv3_domain* d, int mode, const int* nord4, const float* q,
+                        const float* mass, const float* damp, const float* del6_u,
+                        const float* del6_v, const float* rarea, float* fx, float* fy,
+                        void* stream) {
+  return fv3_delnflux_f32(d, mode, nord4, q, mass, damp, del6_u, del6_v, rarea, fx, fy, stream);
 }
 
 int check_dsw_metrics(const fv3_metrics* g, const char* who) {
@@ -805,30 +883,25 @@ int check_dsw_metrics(const fv3_metrics* g, const char* who) {
   return check_a2b_metrics(g, who);
 }
 
-}  // namespace
-
-}  // namespace fv3
-
-extern "C" {
-
-int32_t fv3_d_sw_f64(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_params* p,
-                     const fv3_dsw_state* st, double dt, double* work, int64_t work_slab,
-                     void* stream) {
-  using namespace fv3;
+template <typename T, typename ST>
+int run_d_sw(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_params* p,
+             const ST* state, T dt, T* work, int64_t work_slab, void* stream) {
   const char* who = "fv3_d_sw";
   int rc = check_domain(dom, who);
   if (rc) return rc;
   rc = check_dsw_metrics(g, who);
   if (rc) return rc;
-  FV3_REQUIRE(p && st && work, FV3_E_ARG, "%s: NULL argument", who);
-  FV3_REQUIRE(st->delpc && st->delp && st->ptc && st->pt && st->u && st->v && st->w && st->uc &&
-                  st->vc && st->ua && st->va && st->divgd && st->mfx && st->mfy && st->cx &&
-                  st->cy && st->crx && st->cry && st->xfx && st->yfx && st->q_con &&
-                  st->heat_source && st->diss_est,
+  FV3_REQUIRE(p && state && work, FV3_E_ARG, "%s: NULL argument", who);
+  const DswState<T> st = dsw_state<T>(state);
+  FV3_REQUIRE(st.delpc && st.delp && st.ptc && st.pt && st.u && st.v && st.w && st.uc &&
+                  st.vc && st.ua && st.va && st.divgd && st.mfx && st.mfy && st.cx &&
+                  st.cy && st.crx && st.cry && st.xfx && st.yfx && st.q_con &&
+                  st.heat_source && st.diss_est,
               FV3_E_ARG, "%s: NULL state member", who);
   FV3_REQUIRE(p->damp_vt_deln && p->damp_t_deln && p->damp_w_nosg && p->damp_v_nosg &&
                   p->damp_w && p->damp_vt && p->ke_bg && p->d_con_col && p->d2_divg,
               FV3_E_ARG, "%s: NULL column", who);
+  const DswCols<T> cols = dsw_cols<T>(p);
   const int hords[4] = {p->hord_dp, p->hord_tm, p->hord_mt, p->hord_vt};
   for (int h : hords)
     FV3_REQUIRE(h >= 5 && h <= 8, FV3_E_UNSUPPORTED, "%s: hord %d not in {5, 6, 7, 8}", who, h);
@@ -841,101 +914,116 @@ int32_t fv3_d_sw_f64(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_
   FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)nk1 * dom->sk,
               FV3_E_ARG, "%s: work_slab too small", who);
   hipStream_t s = as_stream(stream);
-  const Met<double> m = to_met<double>(g);
+  const Met<T> m = to_met<T>(g);
   Dom d = to_dom(dom);
   fv3_domain d1 = *dom;  // transport levels: nk + 1
   d1.nk = nk1;
-  double* W[22];
+  T* W[22];
   for (int n = 0; n < 22; ++n) W[n] = work + (int64_t)n * work_slab;
-  double *ucc = W[0], *vcc = W[1], *fx = W[2], *fy = W[3], *gx = W[4], *gy = W[5];
-  double *fx2 = W[6], *fy2 = W[7], *dw = W[8], *heat_s = W[9], *wk = W[10], *vort = W[11];
-  double *ke = W[12], *vxd = W[13], *vyd = W[14], *ut = W[15], *vt = W[16], *qy = W[17];
-  double *a2x = W[18], *a2e = W[20], *vort2 = W[21];  // W[19] is free
-  const double* area = static_cast<const double*>(g->area);
-  const double* dxa = static_cast<const double*>(g->dxa);
-  const double* dya = static_cast<const double*>(g->dya);
-  const double* del6_u = static_cast<const double*>(g->del6_u);
-  const double* del6_v = static_cast<const double*>(g->del6_v);
-  const double* rarea = static_cast<const double*>(g->rarea);
+  T *ucc = W[0], *vcc = W[1], *fx = W[2], *fy = W[3], *gx = W[4], *gy = W[5];
+  T *fx2 = W[6], *fy2 = W[7], *dw = W[8], *heat_s = W[9], *wk = W[10], *vort = W[11];
+  T *ke = W[12], *vxd = W[13], *vyd = W[14], *ut = W[15], *vt = W[16], *qy = W[17];
+  T *a2x = W[18], *a2e = W[20], *vort2 = W[21];  // W[19] is free
+  const T* area = m.area;
+  const T* dxa = m.dxa;
+  const T* dya = m.dya;
+  const T* del6_u = m.del6_u;
+  const T* del6_v = m.del6_v;
+  const T* rarea = m.rarea;
 
   // transport of q: corner copies (x in place, y applied as the transport
   // reads q), fv_tp_2d on nk + 1 levels, then del-n damping of the fluxes on
   // nk levels
-  auto transport = [&](double* q, int hord, const double* mfx_, const double* mfy_,
-                       const int* nord4, const double* damp, const double* mass, double* ox,
-                       double* oy) -> int {
-    int r = fvtp2d_corner_fill_f64(&d1, hord, q, qy, st->crx, st->cry, st->xfx, st->yfx, mfx_,
-                                   mfy_, area, dxa, dya, ox, oy, p->transport_fp32 != 0, stream);
+  auto transport = [&](T* q, int hord, const T* mfx_, const T* mfy_, const int* nord4,
+                       const T* damp, const T* mass, T* ox, T* oy) -> int {
+    int r = dsw_transport(&d1, hord, q, qy, st.crx, st.cry, st.xfx, st.yfx, mfx_, mfy_, area, dxa,
+                          dya, ox, oy, p->transport_fp32 != 0, stream);
     if (r) return r;
     if (nord4) {
-      r = fv3_delnflux_f64(dom, mass ? FV3_DELN_MASS : FV3_DELN_ADD, nord4, q, mass, damp,
-                           del6_u, del6_v, rarea, ox, oy, stream);
+      r = dsw_delnflux(dom, mass ? FV3_DELN_MASS : FV3_DELN_ADD, nord4, q, mass, damp, del6_u,
+                       del6_v, rarea, ox, oy, stream);
     }
     return r;
   };
 
-  rc = run_fxadv<double>(s, d, m, dt, st->uc, st->vc, st->crx, st->cry, st->xfx, st->yfx, ucc,
-                         vcc, who);
+  rc = run_fxadv<T>(s, d, m, dt, st.uc, st.vc, st.crx, st.cry, st.xfx, st.yfx, ucc, vcc, who);
   if (rc) return rc;
-  rc = transport(st->delp, p->hord_dp, nullptr, nullptr, p->nord_v4, p->damp_vt_deln, nullptr,
+  rc = transport(st.delp, p->hord_dp, nullptr, nullptr, p->nord_v4, cols.damp_vt_deln, nullptr,
                  fx, fy);
   if (rc) return rc;
   const int nonhydro = !p->hydrostatic;
   if (nonhydro) {
-    rc = fv3_delnflux_f64(dom, FV3_DELN_STORE, p->nord_w4, st->w, nullptr, p->damp_w_nosg,
-                          del6_u, del6_v, rarea, fx2, fy2, stream);
+    rc = dsw_delnflux(dom, FV3_DELN_STORE, p->nord_w4, st.w, nullptr, cols.damp_w_nosg, del6_u,
+                      del6_v, rarea, fx2, fy2, stream);
     if (rc) return rc;
   }
-  launch_capacitor_heat<double>(s, d, m, dt, nonhydro, p, st, fx, fy, fx2, fy2, dw, heat_s);
+  launch_capacitor_heat<T>(s, d, m, dt, nonhydro, cols, st, fx, fy, fx2, fy2, dw, heat_s);
   FV3_HIP_CHECK_LAUNCH(who);
   if (nonhydro) {
-    rc = transport(st->w, p->hord_vt, fx, fy, nullptr, nullptr, nullptr, gx, gy);
+    rc = transport(st.w, p->hord_vt, fx, fy, nullptr, nullptr, nullptr, gx, gy);
     if (rc) return rc;
-    launch_flux_adjust<double>(s, d, m, st->w, st->delp, gx, gy);
+    launch_flux_adjust<T>(s, d, m, st.w, st.delp, gx, gy);
     FV3_HIP_CHECK_LAUNCH(who);
   }
-  rc = transport(st->q_con, p->hord_dp, fx, fy, p->nord_t4, p->damp_t_deln, st->delp, gx, gy);
+  rc = transport(st.q_con, p->hord_dp, fx, fy, p->nord_t4, cols.damp_t_deln, st.delp, gx, gy);
   if (rc) return rc;
-  launch_flux_adjust<double>(s, d, m, st->q_con, st->delp, gx, gy);
+  launch_flux_adjust<T>(s, d, m, st.q_con, st.delp, gx, gy);
   FV3_HIP_CHECK_LAUNCH(who);
-  rc = transport(st->pt, p->hord_tm, fx, fy, p->nord_v4, p->damp_vt_deln, st->delp, gx, gy);
+  rc = transport(st.pt, p->hord_tm, fx, fy, p->nord_v4, cols.damp_vt_deln, st.delp, gx, gy);
   if (rc) return rc;
-  launch_pt_delp<double>(s, d, m, st->pt, st->delp, gx, gy, fx, fy);
+  launch_pt_delp<T>(s, d, m, st.pt, st.delp, gx, gy, fx, fy);
   FV3_HIP_CHECK_LAUNCH(who);
   switch (p->hord_mt) {
-    case 5: launch_ke_ord<5>(s, d, m, dt, st->uc, st->vc, st->u, st->v, ucc, vcc, ke); break;
-    case 6: launch_ke_ord<6>(s, d, m, dt, st->uc, st->vc, st->u, st->v, ucc, vcc, ke); break;
-    case 7: launch_ke_ord<7>(s, d, m, dt, st->uc, st->vc, st->u, st->v, ucc, vcc, ke); break;
-    default: launch_ke_ord<8>(s, d, m, dt, st->uc, st->vc, st->u, st->v, ucc, vcc, ke); break;
+    case 5: launch_ke_ord<T, 5>(s, d, m, dt, st.uc, st.vc, st.u, st.v, ucc, vcc, ke); break;
+    case 6: launch_ke_ord<T, 6>(s, d, m, dt, st.uc, st.vc, st.u, st.v, ucc, vcc, ke); break;
+    case 7: launch_ke_ord<T, 7>(s, d, m, dt, st.uc, st.vc, st.u, st.v, ucc, vcc, ke); break;
+    default: launch_ke_ord<T, 8>(s, d, m, dt, st.uc, st.vc, st.u, st.v, ucc, vcc, ke); break;
   }
   FV3_HIP_CHECK_LAUNCH(who);
-  launch_vorticity_adjust<double>(s, d, m, st->u, st->v, wk, st->w, st->delp, dw, st->q_con,
-                                  p->damp_w);
+  launch_vorticity_adjust<T>(s, d, m, st.u, st.v, wk, st.w, st.delp, dw, st.q_con, cols.damp_w);
   FV3_HIP_CHECK_LAUNCH(who);
   // divergence damping
   const int k_lo = p->div_k0 < dom->nk ? p->div_k0 : dom->nk;
   if (k_lo > 0) {
-    launch_div_low<double>(s, d, m, k_lo, dt, p->da_min_c, p->dddmp, p->d2_divg, st->u, st->v,
-                           st->ua, st->va, st->uc, st->vc, st->delpc, st->ptc, vort, ke);
+    launch_div_low<T>(s, d, m, k_lo, dt, (T)p->da_min_c, (T)p->dddmp, cols.d2_divg, st.u, st.v,
+                      st.ua, st.va, st.uc, st.vc, st.delpc, st.ptc, vort, ke);
     FV3_HIP_CHECK_LAUNCH(who);
   }
-  rc = run_div_high<double>(s, dom, m, p, dt, wk, st->divgd, st->vc, st->uc, st->delpc, vort, ke,
-                            a2x, a2e, who);
+  rc = run_div_high<T>(s, dom, m, p, cols, dt, wk, st.divgd, st.vc, st.uc, st.delpc, vort, ke,
+                       a2x, a2e, who);
   if (rc) return rc;
-  launch_vort_diff<double>(s, d, m, vort, wk, vxd, vyd, vort2, p->d_con_col);
+  launch_vort_diff<T>(s, d, m, vort, wk, vxd, vyd, vort2, cols.d_con_col);
   FV3_HIP_CHECK_LAUNCH(who);
   rc = transport(vort2, p->hord_vt, nullptr, nullptr, nullptr, nullptr, nullptr, fx, fy);
   if (rc) return rc;
-  launch_uv_from_ke<double>(s, d, m, ke, fx, fy, st->u, st->v);
+  launch_uv_from_ke<T>(s, d, m, ke, fx, fy, st.u, st.v);
   FV3_HIP_CHECK_LAUNCH(who);
-  rc = fv3_delnflux_f64(dom, FV3_DELN_STORE, p->nord_v4, wk, nullptr, p->damp_v_nosg, del6_u,
-                        del6_v, rarea, ut, vt, stream);
+  rc = dsw_delnflux(dom, FV3_DELN_STORE, p->nord_v4, wk, nullptr, cols.damp_v_nosg, del6_u,
+                    del6_v, rarea, ut, vt, stream);
   if (rc) return rc;
-  launch_heat_source<double>(s, d, m, vxd, vyd, ut, vt, st->u, st->v, st->delp, heat_s,
-                             st->heat_source, p->d_con_col, p->d_con > kDconThreshold);
+  launch_heat_source<T>(s, d, m, vxd, vyd, ut, vt, st.u, st.v, st.delp, heat_s, st.heat_source,
+                        cols.d_con_col, p->d_con > kDconThreshold);
   FV3_HIP_CHECK_LAUNCH(who);
-  launch_update_uv<double>(s, d, ut, vt, st->u, st->v, p->damp_vt);
+  launch_update_uv<T>(s, d, ut, vt, st.u, st.v, cols.damp_vt);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
+}
+
+}  // namespace
+
+}  // namespace fv3
+
+extern "C" {
+
+int32_t fv3_d_sw_f64(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_params* p,
+                     const fv3_dsw_state* st, double dt, double* work, int64_t work_slab,
+                     void* stream) {
+  return fv3::run_d_sw<double>(dom, g, p, st, dt, work, work_slab, stream);
+}
+
+int32_t fv3_d_sw_f32(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_params* p,
+                     const fv3_dsw_state_f32* st, float dt, float* work, int64_t work_slab,
+                     void* stream) {
+  return fv3::run_d_sw<float>(dom, g, p, st, dt, work, work_slab, stream);
 }
 }

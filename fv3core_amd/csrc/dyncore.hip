@@ -366,6 +366,312 @@ __global__ void __launch_bounds__(256)
 
 }  // namespace
 
+inline int copy_corners_xy(const fv3_domain* d, const double* q, double* qx, double* qy,
+                           void* stream) {
+  return fv3_copy_corners_xy_f64(d, q, qx, qy, stream);
+}
+
+inline int copy_corners_xy(const fv3_domain* d, const float* q, float* qx, float* qy,
+                           void* stream) {
+  return fv3_copy_corners_xy_f32(d, q, qx, qy, stream);
+}
+
+// p_grad_c_stencil (dyn_core.py:101-145) on [isc, iec+1]^2; the winds and
+// delpc in storage type S (FP32 in the mixed form), the pressure and
+// geopotential terms and the arithmetic in FP64.
+template <typename S>
+int p_grad_c(const fv3_domain* dom, const double* rdxc, const double* rdyc, int hydrostatic,
+             S* uc, S* vc, const S* delpc, const double* pkc, const double* gz, double dt2,
+             void* stream) {
+  const char* who = "fv3_p_grad_c";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(rdxc && rdyc && uc && vc && pkc && gz && (hydrostatic || delpc), FV3_E_ARG,
+              "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const Box bx{d.isc, d.iec + 1, d.jsc, d.jec + 1};
+  launch_box(as_stream(stream), d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const int64_t o2 = at2(d, i, j, t);
+    const int64_t sk = d.sk, sj = d.sj;
+    auto wk = [&](int64_t q) -> double {
+      return hydrostatic ? pkc[q + sk] - pkc[q] : (double)delpc[q];
+    };
+    const double w0 = wk(o);
+    {
+      const int64_t q = o - 1;
+      uc[o] = (S)((double)uc[o] + dt2 * rdxc[o2] / (wk(q) + w0) *
+                                      ((gz[q + sk] - gz[o]) * (pkc[o + sk] - pkc[q]) +
+                                       (gz[q] - gz[o + sk]) * (pkc[q + sk] - pkc[o])));
+    }
+    {
+      const int64_t q = o - sj;
+      vc[o] = (S)((double)vc[o] + dt2 * rdyc[o2] / (wk(q) + w0) *
+                                      ((gz[q + sk] - gz[o]) * (pkc[o + sk] - pkc[q]) +
+                                       (gz[q] - gz[o + sk]) * (pkc[q + sk] - pkc[o])));
+    }
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+template <typename T>
+int ray_fast(const fv3_domain* dom, int hydrostatic, const T* rf, const T* dp, const T* dm,
+             const int32_t* use_rf, const int32_t* use_nudge, T* u, T* v, T* w, void* stream) {
+  const char* who = "fv3_ray_fast";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(rf && dp && dm && use_rf && use_nudge && u && v && (hydrostatic || w), FV3_E_ARG,
+              "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const int nc = (d.iec - d.isc + 2) * (d.jec - d.jsc + 2);
+  hipLaunchKernelGGL((ray_fast_kernel<T>), dim3((nc + 255) / 256, d.ntiles), dim3(256), 0,
+                     as_stream(stream), d, hydrostatic ? 0 : 1, rf, dp, dm, use_rf, use_nudge, u,
+                     v, w);
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+template <typename T>
+int del2cubed(const fv3_domain* dom, const T* del6_u, const T* del6_v, const T* rarea, int nmax,
+              T cd, T* qdel, T* work, int64_t work_slab, void* stream) {
+  const char* who = "fv3_del2cubed";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(del6_u && del6_v && rarea && qdel && work, FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)dom->nk * dom->sk,
+              FV3_E_ARG, "%s: work_slab too small", who);
+  Dom d = to_dom(dom);
+  hipStream_t s = as_stream(stream);
+  T* q = work;
+  T* qy = work + work_slab;
+  const int ntimes = nmax < 3 ? nmax : 3;
+  const Box full{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
+  for (int n = 0; n < ntimes; ++n) {
+    const int nt = ntimes - (n + 1);
+    // corner_fill (del2cubed.py:30-62): q = qdel with three-cell averages
+    // at the 12 points around the tile corners
+    launch_box(s, d, full, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      const int64_t sj = d.sj;
+      const T third = T(1.0) / T(3.0);
+      T val = qdel[o];
+      const bool w = i == d.is || i == d.is - 1, e = i == d.ie || i == d.ie + 1;
+      const bool so = j == d.js || j == d.js - 1, no = j == d.je || j == d.je + 1;
+      if ((w || e) && (so || no)) {
+        const int ci = w ? d.is : d.ie, cj = so ? d.js : d.je;
+        const int ox = w ? -1 : 1, oy = so ? -1 : 1;  // outward directions
+        if (i == ci && j == cj) {
+          val = (qdel[o] + qdel[o + ox] + qdel[o + oy * sj]) * third;
+        } else if (i == ci + ox && j == cj) {
+          val = (qdel[o - ox] + qdel[o] + qdel[o - ox + oy * sj]) * third;
+        } else if (i == ci && j == cj + oy) {
+          val = (qdel[o - oy * sj] + qdel[o + ox - oy * sj] + qdel[o]) * third;
+        }
+      }
+      q[o] = val;
+    });
+    FV3_HIP_CHECK_LAUNCH(who);
+    const T* qxp = q;
+    const T* qyp = q;
+    if (nt > 0) {
+      rc = copy_corners_xy(dom, q, q, qy, stream);
+      if (rc) return rc;
+      qyp = qy;
+    }
+    // copy to qdel and the flux update on the compute domain + nt halo
+    launch_box(s, d, full, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      T val = qyp[o];
+      if (i >= d.isc - nt && i <= d.iec + nt && j >= d.jsc - nt && j <= d.jec + nt) {
+        const int64_t o2 = at2(d, i, j, t);
+        const int64_t sj = d.sj, sj2 = d.sj2;
+        const T fx0 = del6_v[o2] * (qxp[o - 1] - qxp[o]);
+        const T fx1 = del6_v[o2 + 1] * (qxp[o] - qxp[o + 1]);
+        const T fy0 = del6_u[o2] * (qyp[o - sj] - qyp[o]);
+        const T fy1 = del6_u[o2 + sj2] * (qyp[o] - qyp[o + sj]);
+        val = val + cd * rarea[o2] * (fx0 - fx1 + fy0 - fy1);
+      }
+      qdel[o] = val;
+    });
+    FV3_HIP_CHECK_LAUNCH(who);
+  }
+  return FV3_OK;
+}
+
+// compute_pkz_tempadjust (temperature_adjust.py:8-44): delp, cappa,
+// heat_source and pt in storage type S, delz and pkz and the arithmetic FP64.
+template <typename S>
+int pkz_tempadjust(const fv3_domain* dom, const S* delp, const double* delz, const S* cappa,
+                   const S* heat_source, S* pt, double* pkz, double delt_time_factor,
+                   void* stream) {
+  const char* who = "fv3_pkz_tempadjust";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(delp && delz && cappa && heat_source && pt && pkz, FV3_E_ARG, "%s: NULL pointer",
+              who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  const Box comp{d.isc, d.iec, d.jsc, d.jec};
+  const double rdg = -287.05 / 9.80665;
+  const double cv_air = 1004.6 - 287.05;
+  launch_box(as_stream(stream), d, comp, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const double c = cappa[o];
+    const double dp = delp[o], ptv = pt[o];
+    const double pk = pow(rdg * dp / delz[o] * ptv, c / (1.0 - c));
+    pkz[o] = pk;
+    const double dtmp = (double)heat_source[o] / (cv_air * dp);
+    const double f = k == 0 ? delt_time_factor * 0.1
+                            : (k == 1 ? delt_time_factor * 0.5 : delt_time_factor);
+    const double deltmin = sgn(fmin(f, fabs(dtmp)), dtmp);
+    pt[o] = (S)(ptv + deltmin / pk);
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+// NonHydrostaticPressureGradient.__call__ (nh_p_grad.py:147-206): the
+// winds in storage type S (FP32 in the mixed form), the pressures, heights,
+// delp and the arithmetic FP64.
+template <typename S>
+int nh_p_grad(const fv3_domain* dom, const fv3_metrics* g, S* u, S* v, double* pp, double* gz,
+              double* pk3, const double* delp, double dt, double ptk, double* work,
+              int64_t work_slab, void* stream) {
+  const char* who = "fv3_nh_p_grad";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  rc = check_a2b_metrics(g, who);
+  if (rc) return rc;
+  FV3_REQUIRE(g->rdx && g->rdy, FV3_E_ARG, "%s: rdx/rdy are NULL", who);
+  FV3_REQUIRE(u && v && pp && gz && pk3 && delp && work, FV3_E_ARG, "%s: NULL pointer", who);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  const int nz = dom->nk;
+  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)(nz + 1) * dom->sk,
+              FV3_E_ARG, "%s: work_slab too small", who);
+  hipStream_t s = as_stream(stream);
+  const Met<double> m = to_met<double>(g);
+  double *wk = work, *wk1 = work + work_slab, *qx = work + 2 * work_slab;
+  double* edges = work + 4 * work_slab;
+  // a2b_k1 on pp and pk3 (levels 1..nz), a2b_kbuffer on gz (0..nz), replace
+  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pp, wk1, qx, edges, 1, pp, who);
+  if (rc) return rc;
+  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pk3, wk1, qx, edges, 1, pk3, who);
+  if (rc) return rc;
+  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz + 1, gz, wk1, qx, edges, 1, gz, who);
+  if (rc) return rc;
+  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz, delp, wk1, qx, edges, 0, nullptr, who);
+  if (rc) return rc;
+  Dom d = to_dom(dom);
+  // set_k0_and_calc_wk on [isc, iec+1]^2
+  const Box bc{d.isc, d.iec + 1, d.jsc, d.jec + 1};
+  launch_box(s, d, bc, 0, nz, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    if (k == 0) {
+      pp[o] = 0.0;
+      pk3[o] = ptk;
+    }
+    wk[o] = pk3[o + d.sk] - pk3[o];
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  // calc_u on [isc, iec] x [jsc, jec+1] and calc_v on [isc, iec+1] x [jsc, jec]
+  launch_box(s, d, bc, 0, nz, [=] __device__(int i, int j, int k, int t) {
+    const int64_t o = at3(d, i, j, k, t);
+    const int64_t o2 = at2(d, i, j, t);
+    const int64_t sk = d.sk;
+    if (i <= d.iec) {
+      const int64_t q = o + 1;
+      const double du = dt / (wk[o] + wk[q]) *
+                        ((gz[o + sk] - gz[q]) * (pk3[q + sk] - pk3[o]) +
+                         (gz[o] - gz[q + sk]) * (pk3[o + sk] - pk3[q]));
+      u[o] = (S)(((double)u[o] + du +
+              dt / (wk1[o] + wk1[q]) *
+                  ((gz[o + sk] - gz[q]) * (pp[q + sk] - pp[o]) +
+                   (gz[o] - gz[q + sk]) * (pp[o + sk] - pp[q]))) *
+             m.rdx[o2]);
+    }
+    if (j <= d.jec) {
+      const int64_t q = o + d.sj;
+      const double dv = dt / (wk[o] + wk[q]) *
+                        ((gz[o + sk] - gz[q]) * (pk3[q + sk] - pk3[o]) +
+                         (gz[o] - gz[q + sk]) * (pk3[o + sk] - pk3[q]));
+      v[o] = (S)(((double)v[o] + dv +
+              dt / (wk1[o] + wk1[q]) *
+                  ((gz[o + sk] - gz[q]) * (pp[q + sk] - pp[o]) +
+                   (gz[o] - gz[q + sk]) * (pp[o + sk] - pp[q]))) *
+             m.rdy[o2]);
+    }
+  });
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
+template <typename T>
+int dyncore_column(const fv3_domain* dom, int op, double scalar, const T* a, const T* b, T* out,
+                   void* stream) {
+  const char* who = "fv3_dyncore_column";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(out && (op == 4 || a), FV3_E_ARG, "%s: NULL pointer", who);
+  FV3_REQUIRE(op >= 0 && op <= 4, FV3_E_ARG, "%s: unknown op %d", who, op);
+  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
+  Dom d = to_dom(dom);
+  hipStream_t s = as_stream(stream);
+  if (op == 0) {
+    // set_gz (dyn_core.py:80-85): out = gz on nk levels, a = zs (IJ), b = delz
+    FV3_REQUIRE(b, FV3_E_ARG, "%s: NULL delz", who);
+    const Box comp{d.isc, d.iec, d.jsc, d.jec};
+    launch_box(s, d, comp, 0, 1, [=] __device__(int i, int j, int, int t) {
+      int64_t o = at3(d, i, j, d.nk - 1, t);
+      T g = a[at2(d, i, j, t)];
+      out[o] = g;
+      for (int k = d.nk - 2; k >= 0; --k) {
+        o -= d.sk;
+        g = g - b[o];
+        out[o] = g;
+      }
+    });
+  } else if (op == 1) {
+    // set_pem (dyn_core.py:88-93): out = pem, a = delp, scalar = ptop
+    const Box bx{d.isc - 1, d.iec + 1, d.jsc - 1, d.jec + 1};
+    launch_box(s, d, bx, 0, 1, [=] __device__(int i, int j, int, int t) {
+      int64_t o = at3(d, i, j, 0, t);
+      T p = (T)scalar;
+      out[o] = p;
+      for (int k = 1; k < d.nk; ++k) {
+        o += d.sk;
+        p = p + a[o];
+        out[o] = p;
+      }
+    });
+  } else if (op == 2) {
+    // compute_geopotential (dyn_core.py:96-98) on compute + 2 halo: out = a * GRAV
+    const Box bx{d.isc - 2, d.iec + 2, d.jsc - 2, d.jec + 2};
+    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      out[o] = a[o] * T(9.80665);
+    });
+  } else if (op == 3) {
+    // copy_defn on the full domain
+    const Box bx{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
+    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      out[o] = a[o];
+    });
+  } else {
+    // fill the full domain with a value
+    const Box bx{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
+    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      out[at3(d, i, j, k, t)] = (T)scalar;
+    });
+  }
+  FV3_HIP_CHECK_LAUNCH(who);
+  return FV3_OK;
+}
+
 // (ring column, level) pairs of a ring of the given width around the box
 static int64_t ring_threads(const Dom& d, int width) {
   const int ni = d.iec - d.isc + 1, nj = d.jec - d.jsc + 1;
@@ -464,73 +770,15 @@ int32_t fv3_update_dz_d_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t
 int32_t fv3_nh_p_grad_f64(const fv3_domain* dom, const fv3_metrics* g, double* u, double* v,
                           double* pp, double* gz, double* pk3, const double* delp, double dt,
                           double ptk, double* work, int64_t work_slab, void* stream) {
-  using namespace fv3;
-  const char* who = "fv3_nh_p_grad";
-  int rc = check_domain(dom, who);
-  if (rc) return rc;
-  rc = check_a2b_metrics(g, who);
-  if (rc) return rc;
-  FV3_REQUIRE(g->rdx && g->rdy, FV3_E_ARG, "%s: rdx/rdy are NULL", who);
-  FV3_REQUIRE(u && v && pp && gz && pk3 && delp && work, FV3_E_ARG, "%s: NULL pointer", who);
-  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
-  const int nz = dom->nk;
-  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)(nz + 1) * dom->sk,
-              FV3_E_ARG, "%s: work_slab too small", who);
-  hipStream_t s = as_stream(stream);
-  const Met<double> m = to_met<double>(g);
-  double *wk = work, *wk1 = work + work_slab, *qx = work + 2 * work_slab;
-  double* edges = work + 4 * work_slab;
-  // a2b_k1 on pp and pk3 (levels 1..nz), a2b_kbuffer on gz (0..nz), replace
-  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pp, wk1, qx, edges, 1, pp, who);
-  if (rc) return rc;
-  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pk3, wk1, qx, edges, 1, pk3, who);
-  if (rc) return rc;
-  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz + 1, gz, wk1, qx, edges, 1, gz, who);
-  if (rc) return rc;
-  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz, delp, wk1, qx, edges, 0, nullptr, who);
-  if (rc) return rc;
-  Dom d = to_dom(dom);
-  // set_k0_and_calc_wk on [isc, iec+1]^2
-  const Box bc{d.isc, d.iec + 1, d.jsc, d.jec + 1};
-  launch_box(s, d, bc, 0, nz, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
-    if (k == 0) {
-      pp[o] = 0.0;
-      pk3[o] = ptk;
-    }
-    wk[o] = pk3[o + d.sk] - pk3[o];
-  });
-  FV3_HIP_CHECK_LAUNCH(who);
-  // calc_u on [isc, iec] x [jsc, jec+1] and calc_v on [isc, iec+1] x [jsc, jec]
-  launch_box(s, d, bc, 0, nz, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
-    const int64_t o2 = at2(d, i, j, t);
-    const int64_t sk = d.sk;
-    if (i <= d.iec) {
-      const int64_t q = o + 1;
-      const double du = dt / (wk[o] + wk[q]) *
-                        ((gz[o + sk] - gz[q]) * (pk3[q + sk] - pk3[o]) +
-                         (gz[o] - gz[q + sk]) * (pk3[o + sk] - pk3[q]));
-      u[o] = (u[o] + du +
-              dt / (wk1[o] + wk1[q]) *
-                  ((gz[o + sk] - gz[q]) * (pp[q + sk] - pp[o]) +
-                   (gz[o] - gz[q + sk]) * (pp[o + sk] - pp[q]))) *
-             m.rdx[o2];
-    }
-    if (j <= d.jec) {
-      const int64_t q = o + d.sj;
-      const double dv = dt / (wk[o] + wk[q]) *
-                        ((gz[o + sk] - gz[q]) * (pk3[q + sk] - pk3[o]) +
-                         (gz[o] - gz[q + sk]) * (pk3[o + sk] - pk3[q]));
-      v[o] = (v[o] + dv +
-              dt / (wk1[o] + wk1[q]) *
-                  ((gz[o + sk] - gz[q]) * (pp[q + sk] - pp[o]) +
-                   (gz[o] - gz[q + sk]) * (pp[o + sk] - pp[q]))) *
-             m.rdy[o2];
-    }
-  });
-  FV3_HIP_CHECK_LAUNCH(who);
-  return FV3_OK;
+  return fv3::nh_p_grad<double>(dom, g, u, v, pp, gz, pk3, delp, dt, ptk, work, work_slab,
+                                stream);
+}
+
+int32_t fv3_nh_p_grad_mixed(const fv3_domain* dom, const fv3_metrics* g, float* u, float* v,
+                            double* pp, double* gz, double* pk3, const double* delp, double dt,
+                            double ptk, double* work, int64_t work_slab, void* stream) {
+  return fv3::nh_p_grad<float>(dom, g, u, v, pp, gz, pk3, delp, dt, ptk, work, work_slab,
+                               stream);
 }
 
 int32_t fv3_pk3_halo_f64(const fv3_domain* dom, double* pk3, const double* delp, double ptop,
@@ -568,216 +816,63 @@ int32_t fv3_pe_halo_f64(const fv3_domain* dom, double* pe, const double* delp, d
 int32_t fv3_p_grad_c_f64(const fv3_domain* dom, const double* rdxc, const double* rdyc,
                          int32_t hydrostatic, double* uc, double* vc, const double* delpc,
                          const double* pkc, const double* gz, double dt2, void* stream) {
-  using namespace fv3;
-  const char* who = "fv3_p_grad_c";
-  int rc = check_domain(dom, who);
-  if (rc) return rc;
-  FV3_REQUIRE(rdxc && rdyc && uc && vc && pkc && gz && (hydrostatic || delpc), FV3_E_ARG,
-              "%s: NULL pointer", who);
-  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
-  Dom d = to_dom(dom);
-  const Box bx{d.isc, d.iec + 1, d.jsc, d.jec + 1};
-  launch_box(as_stream(stream), d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
-    const int64_t o2 = at2(d, i, j, t);
-    const int64_t sk = d.sk, sj = d.sj;
-    auto wk = [&](int64_t q) { return hydrostatic ? pkc[q + sk] - pkc[q] : delpc[q]; };
-    const double w0 = wk(o);
-    {
-      const int64_t q = o - 1;
-      uc[o] = uc[o] + dt2 * rdxc[o2] / (wk(q) + w0) *
-                          ((gz[q + sk] - gz[o]) * (pkc[o + sk] - pkc[q]) +
-                           (gz[q] - gz[o + sk]) * (pkc[q + sk] - pkc[o]));
-    }
-    {
-      const int64_t q = o - sj;
-      vc[o] = vc[o] + dt2 * rdyc[o2] / (wk(q) + w0) *
-                          ((gz[q + sk] - gz[o]) * (pkc[o + sk] - pkc[q]) +
-                           (gz[q] - gz[o + sk]) * (pkc[q + sk] - pkc[o]));
-    }
-  });
-  FV3_HIP_CHECK_LAUNCH(who);
-  return FV3_OK;
+  return fv3::p_grad_c<double>(dom, rdxc, rdyc, hydrostatic, uc, vc, delpc, pkc, gz, dt2, stream);
+}
+
+int32_t fv3_p_grad_c_mixed(const fv3_domain* dom, const double* rdxc, const double* rdyc,
+                           int32_t hydrostatic, float* uc, float* vc, const float* delpc,
+                           const double* pkc, const double* gz, double dt2, void* stream) {
+  return fv3::p_grad_c<float>(dom, rdxc, rdyc, hydrostatic, uc, vc, delpc, pkc, gz, dt2, stream);
 }
 
 int32_t fv3_ray_fast_f64(const fv3_domain* dom, int32_t hydrostatic, const double* rf,
                          const double* dp, const double* dm, const int32_t* use_rf,
                          const int32_t* use_nudge, double* u, double* v, double* w,
                          void* stream) {
-  using namespace fv3;
-  const char* who = "fv3_ray_fast";
-  int rc = check_domain(dom, who);
-  if (rc) return rc;
-  FV3_REQUIRE(rf && dp && dm && use_rf && use_nudge && u && v && (hydrostatic || w), FV3_E_ARG,
-              "%s: NULL pointer", who);
-  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
-  Dom d = to_dom(dom);
-  const int nc = (d.iec - d.isc + 2) * (d.jec - d.jsc + 2);
-  hipLaunchKernelGGL((ray_fast_kernel<double>), dim3((nc + 255) / 256, d.ntiles), dim3(256), 0,
-                     as_stream(stream), d, hydrostatic ? 0 : 1, rf, dp, dm, use_rf, use_nudge, u,
-                     v, w);
-  FV3_HIP_CHECK_LAUNCH(who);
-  return FV3_OK;
+  return fv3::ray_fast<double>(dom, hydrostatic, rf, dp, dm, use_rf, use_nudge, u, v, w, stream);
+}
+
+int32_t fv3_ray_fast_f32(const fv3_domain* dom, int32_t hydrostatic, const float* rf,
+                         const float* dp, const float* dm, const int32_t* use_rf,
+                         const int32_t* use_nudge, float* u, float* v, float* w, void* stream) {
+  return fv3::ray_fast<float>(dom, hydrostatic, rf, dp, dm, use_rf, use_nudge, u, v, w, stream);
 }
 
 int32_t fv3_del2cubed_f64(const fv3_domain* dom, const double* del6_u, const double* del6_v,
                           const double* rarea, int32_t nmax, double cd, double* qdel,
                           double* work, int64_t work_slab, void* stream) {
-  using namespace fv3;
-  const char* who = "fv3_del2cubed";
-  int rc = check_domain(dom, who);
-  if (rc) return rc;
-  FV3_REQUIRE(del6_u && del6_v && rarea && qdel && work, FV3_E_ARG, "%s: NULL pointer", who);
-  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
-  FV3_REQUIRE(work_slab >= (int64_t)(dom->ntiles - 1) * dom->st + (int64_t)dom->nk * dom->sk,
-              FV3_E_ARG, "%s: work_slab too small", who);
-  Dom d = to_dom(dom);
-  hipStream_t s = as_stream(stream);
-  double* q = work;
-  double* qy = work + work_slab;
-  const int ntimes = nmax < 3 ? nmax : 3;
-  const Box full{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
-  for (int n = 0; n < ntimes; ++n) {
-    const int nt = ntimes - (n + 1);
-    // corner_fill (del2cubed.py:30-62): q = qdel with three-cell averages
-    // at the 12 points around the tile corners
-    launch_box(s, d, full, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-      const int64_t o = at3(d, i, j, k, t);
-      const int64_t sj = d.sj;
-      const double third = 1.0 / 3.0;
-      double val = qdel[o];
-      const bool w = i == d.is || i == d.is - 1, e = i == d.ie || i == d.ie + 1;
-      const bool so = j == d.js || j == d.js - 1, no = j == d.je || j == d.je + 1;
-      if ((w || e) && (so || no)) {
-        const int ci = w ? d.is : d.ie, cj = so ? d.js : d.je;
-        const int ox = w ? -1 : 1, oy = so ? -1 : 1;  // outward directions
-        if (i == ci && j == cj) {
-          val = (qdel[o] + qdel[o + ox] + qdel[o + oy * sj]) * third;
-        } else if (i == ci + ox && j == cj) {
-          val = (qdel[o - ox] + qdel[o] + qdel[o - ox + oy * sj]) * third;
-        } else if (i == ci && j == cj + oy) {
-          val = (qdel[o - oy * sj] + qdel[o + ox - oy * sj] + qdel[o]) * third;
-        }
-      }
-      q[o] = val;
-    });
-    FV3_HIP_CHECK_LAUNCH(who);
-    const double* qxp = q;
-    const double* qyp = q;
-    if (nt > 0) {
-      rc = fv3_copy_corners_xy_f64(dom, q, q, qy, stream);
-      if (rc) return rc;
-      qyp = qy;
-    }
-    // copy to qdel and the flux update on the compute domain + nt halo
-    launch_box(s, d, full, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-      const int64_t o = at3(d, i, j, k, t);
-      double val = qyp[o];
-      if (i >= d.isc - nt && i <= d.iec + nt && j >= d.jsc - nt && j <= d.jec + nt) {
-        const int64_t o2 = at2(d, i, j, t);
-        const int64_t sj = d.sj, sj2 = d.sj2;
-        const double fx0 = del6_v[o2] * (qxp[o - 1] - qxp[o]);
-        const double fx1 = del6_v[o2 + 1] * (qxp[o] - qxp[o + 1]);
-        const double fy0 = del6_u[o2] * (qyp[o - sj] - qyp[o]);
-        const double fy1 = del6_u[o2 + sj2] * (qyp[o] - qyp[o + sj]);
-        val = val + cd * rarea[o2] * (fx0 - fx1 + fy0 - fy1);
-      }
-      qdel[o] = val;
-    });
-    FV3_HIP_CHECK_LAUNCH(who);
-  }
-  return FV3_OK;
+  return fv3::del2cubed<double>(dom, del6_u, del6_v, rarea, nmax, cd, qdel, work, work_slab,
+                                stream);
+}
+
+int32_t fv3_del2cubed_f32(const fv3_domain* dom, const float* del6_u, const float* del6_v,
+                          const float* rarea, int32_t nmax, float cd, float* qdel, float* work,
+                          int64_t work_slab, void* stream) {
+  return fv3::del2cubed<float>(dom, del6_u, del6_v, rarea, nmax, cd, qdel, work, work_slab,
+                               stream);
 }
 
 int32_t fv3_pkz_tempadjust_f64(const fv3_domain* dom, const double* delp, const double* delz,
                                const double* cappa, const double* heat_source, double* pt,
                                double* pkz, double delt_time_factor, void* stream) {
-  using namespace fv3;
-  const char* who = "fv3_pkz_tempadjust";
-  int rc = check_domain(dom, who);
-  if (rc) return rc;
-  FV3_REQUIRE(delp && delz && cappa && heat_source && pt && pkz, FV3_E_ARG, "%s: NULL pointer",
-              who);
-  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
-  Dom d = to_dom(dom);
-  const Box comp{d.isc, d.iec, d.jsc, d.jec};
-  const double rdg = -287.05 / 9.80665;
-  const double cv_air = 1004.6 - 287.05;
-  launch_box(as_stream(stream), d, comp, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
-    const double c = cappa[o];
-    const double pk = pow(rdg * delp[o] / delz[o] * pt[o], c / (1.0 - c));
-    pkz[o] = pk;
-    const double dtmp = heat_source[o] / (cv_air * delp[o]);
-    const double f = k == 0 ? delt_time_factor * 0.1
-                            : (k == 1 ? delt_time_factor * 0.5 : delt_time_factor);
-    const double deltmin = sgn(fmin(f, fabs(dtmp)), dtmp);
-    pt[o] = pt[o] + deltmin / pk;
-  });
-  FV3_HIP_CHECK_LAUNCH(who);
-  return FV3_OK;
+  return fv3::pkz_tempadjust<double>(dom, delp, delz, cappa, heat_source, pt, pkz,
+                                     delt_time_factor, stream);
+}
+
+int32_t fv3_pkz_tempadjust_mixed(const fv3_domain* dom, const float* delp, const double* delz,
+                                 const float* cappa, const float* heat_source, float* pt,
+                                 double* pkz, double delt_time_factor, void* stream) {
+  return fv3::pkz_tempadjust<float>(dom, delp, delz, cappa, heat_source, pt, pkz,
+                                    delt_time_factor, stream);
 }
 
 int32_t fv3_dyncore_column_f64(const fv3_domain* dom, int32_t op, double scalar, const double* a,
                                const double* b, double* out, void* stream) {
-  using namespace fv3;
-  const char* who = "fv3_dyncore_column";
-  int rc = check_domain(dom, who);
-  if (rc) return rc;
-  FV3_REQUIRE(out && (op == 4 || a), FV3_E_ARG, "%s: NULL pointer", who);
-  FV3_REQUIRE(op >= 0 && op <= 4, FV3_E_ARG, "%s: unknown op %d", who, op);
-  if (dom->nk == 0 || dom->ntiles == 0) return FV3_OK;
-  Dom d = to_dom(dom);
-  hipStream_t s = as_stream(stream);
-  if (op == 0) {
-    // set_gz (dyn_core.py:80-85): out = gz on nk levels, a = zs (IJ), b = delz
-    FV3_REQUIRE(b, FV3_E_ARG, "%s: NULL delz", who);
-    const Box comp{d.isc, d.iec, d.jsc, d.jec};
-    launch_box(s, d, comp, 0, 1, [=] __device__(int i, int j, int, int t) {
-      int64_t o = at3(d, i, j, d.nk - 1, t);
-      double g = a[at2(d, i, j, t)];
-      out[o] = g;
-      for (int k = d.nk - 2; k >= 0; --k) {
-        o -= d.sk;
-        g = g - b[o];
-        out[o] = g;
-      }
-    });
-  } else if (op == 1) {
-    // set_pem (dyn_core.py:88-93): out = pem, a = delp, scalar = ptop
-    const Box bx{d.isc - 1, d.iec + 1, d.jsc - 1, d.jec + 1};
-    launch_box(s, d, bx, 0, 1, [=] __device__(int i, int j, int, int t) {
-      int64_t o = at3(d, i, j, 0, t);
-      double p = scalar;
-      out[o] = p;
-      for (int k = 1; k < d.nk; ++k) {
-        o += d.sk;
-        p = p + a[o];
-        out[o] = p;
-      }
-    });
-  } else if (op == 2) {
-    // compute_geopotential (dyn_core.py:96-98) on compute + 2 halo: out = a * GRAV
-    const Box bx{d.isc - 2, d.iec + 2, d.jsc - 2, d.jec + 2};
-    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-      const int64_t o = at3(d, i, j, k, t);
-      out[o] = a[o] * 9.80665;
-    });
-  } else if (op == 3) {
-    // copy_defn on the full domain
-    const Box bx{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
-    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-      const int64_t o = at3(d, i, j, k, t);
-      out[o] = a[o];
-    });
-  } else {
-    // fill the full domain with a value
-    const Box bx{d.isc - 3, d.iec + 3, d.jsc - 3, d.jec + 3};
-    launch_box(s, d, bx, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-      out[at3(d, i, j, k, t)] = scalar;
-    });
-  }
-  FV3_HIP_CHECK_LAUNCH(who);
-  return FV3_OK;
+  return fv3::dyncore_column<double>(dom, op, scalar, a, b, out, stream);
+}
+
+int32_t fv3_dyncore_column_f32(const fv3_domain* dom, int32_t op, double scalar, const float* a,
+                               const float* b, float* out, void* stream) {
+  return fv3::dyncore_column<float>(dom, op, scalar, a, b, out, stream);
 }
 }

@@ -946,6 +946,15 @@ int fvtp2d_corner_fill_f64(const fv3_domain* dom, int hord, double* q, double* q
                                                    xmf, ymf, area, dxa, dya, fx, fy, stream);
 }
 
+int fvtp2d_corner_fill_f32(const fv3_domain* dom, int hord, float* q, float* qy_scratch,
+                           const float* crx, const float* cry, const float* xfx,
+                           const float* yfx, const float* xmf, const float* ymf,
+                           const float* area, const float* dxa, const float* dya, float* fx,
+                           float* fy, void* stream) {
+  return fvtp2d_corner_fill_launch<float, float>(dom, hord, q, qy_scratch, crx, cry, xfx, yfx,
+                                                 xmf, ymf, area, dxa, dya, fx, fy, stream);
+}
+
 }  // namespace fv3
 
 extern "C" {

@@ -27,6 +27,13 @@ int fvtp2d_corner_fill_f64(const fv3_domain* dom, int hord, double* q, double* q
                            const double* area, const double* dxa, const double* dya, double* fx,
                            double* fy, bool fp32_arithmetic, void* stream);
 
+// The same on FP32 storages with FP32 arithmetic.
+int fvtp2d_corner_fill_f32(const fv3_domain* dom, int hord, float* q, float* qy_scratch,
+                           const float* crx, const float* cry, const float* xfx,
+                           const float* yfx, const float* xmf, const float* ymf,
+                           const float* area, const float* dxa, const float* dya, float* fx,
+                           float* fy, void* stream);
+
 // fv_tp_2d of n scalars (<= FV3_MAX_TRACERS) sharing the Courant numbers,
 // area and mass fluxes, in one launch after one launch of their x corner
 // fills in place (the corner handling of fvtp2d_corner_fill_f64).  q, fx, fy:

@@ -52,11 +52,16 @@ struct Riem3Opts {
 // z = gz (geopotential, written at the end), pt = ptc, w = w3 (read only);
 // RiemannSolver3: dp = delp, z = zh (height, written at the end), w in/out,
 // plus the pressure outputs.
+// S is the storage type of cappa, q_con, delp, pt and w: FP64, or FP32 in
+// the mixed-precision acoustic step (loaded into FP64, w stored back
+// rounded); the solve itself is always FP64.
+template <typename S>
 struct RiemFields {
-  const double *cappa, *qc, *dp, *pt;
+  const S *cappa, *qc, *dp, *pt;
   const double* surf;  // hs (C) or zs (3), IJ
   const double* wsd;   // ws (C) or wsd (3), IJ
-  double *z, *w;
+  double* z;
+  S* w;
   double* pef;                                  // C only
   double *delz, *pe, *ppe, *pk3, *pk, *peln;    // 3 only
   double* work;
@@ -66,9 +71,9 @@ struct RiemFields {
 // C = true: RiemannSolverC (riem_solver_c.py:13-64); false: RiemannSolver3
 // (riem_solver3.py:24-103).  The sim1 arithmetic (sim1_solver.py:11-101) is
 // shared and evaluated in the stencils' order.
-template <bool C>
+template <bool C, typename S>
 __global__ void __launch_bounds__(RW)
-    riem_kernel(Dom d, Box2 bx, Riem3Opts o, double dt, double p_fac, RiemFields f) {
+    riem_kernel(Dom d, Box2 bx, Riem3Opts o, double dt, double p_fac, RiemFields<S> f) {
   extern __shared__ double lds_y[];
   const int64_t n = (int64_t)blockIdx.x * RW + threadIdx.x;
   if (n >= (int64_t)bx.nib * bx.njb) return;
@@ -84,12 +89,12 @@ __global__ void __launch_bounds__(RW)
   double* const X = f.work + c;                 // gam / aa / gam2 / new pe
   double* const PEM = f.work + f.slab + c;      // pem (interfaces)
   double* const PM = f.work + 2 * f.slab + c;   // pm (layers)
-  const double* const CP = f.cappa + c;
-  const double* const QC = f.qc + c;
-  const double* const DP = f.dp + c;
-  const double* const PT = f.pt + c;
+  const S* const CP = f.cappa + c;
+  const S* const QC = f.qc + c;
+  const S* const DP = f.dp + c;
+  const S* const PT = f.pt + c;
   double* const Z = f.z + c;
-  double* const W = f.w + c;
+  S* const W = f.w + c;
   const double ptop = o.ptop;
   const double t1g = 2.0 * dt * dt;
   const double rdt = 1.0 / dt;
@@ -319,7 +324,7 @@ __global__ void __launch_bounds__(RW)
           if constexpr (C) {
             f.pef[c + at(k)] = pe_run + pem;
           } else {
-            W[at(k - 1)] = wk;
+            W[at(k - 1)] = (S)wk;
             const double peln_run = log(pem);
             const double pk3k = o.use_logp ? peln_run : exp(o.kappa * peln_run);
             f.pk3[c + at(k)] = pk3k;
@@ -404,16 +409,11 @@ static int check_riem(const fv3_domain* dom, const char* who, int i1, int i2, in
   return FV3_OK;
 }
 
-}  // namespace fv3
-
-extern "C" {
-
-int32_t fv3_riem_solver_c_f64(const fv3_domain* dom, double dt2, double ptop, double p_fac,
-                              const double* cappa, const double* hs, const double* ws,
-                              const double* ptc, const double* q_con, const double* delpc,
-                              double* gz, double* pef, const double* w3, double* work,
-                              int64_t work_slab, void* stream) {
-  using namespace fv3;
+template <typename S>
+int riem_solver_c(const fv3_domain* dom, double dt2, double ptop, double p_fac, const S* cappa,
+                  const double* hs, const double* ws, const S* ptc, const S* q_con,
+                  const S* delpc, double* gz, double* pef, const S* w3, double* work,
+                  int64_t work_slab, void* stream) {
   const char* who = "fv3_riem_solver_c";
   if (!dom) {
     set_error("%s: domain is NULL", who);
@@ -429,7 +429,7 @@ int32_t fv3_riem_solver_c_f64(const fv3_domain* dom, double dt2, double ptop, do
   Dom d = to_dom(dom);
   Riem3Opts o{};
   o.ptop = ptop;
-  RiemFields f{};
+  RiemFields<S> f{};
   f.cappa = cappa;
   f.qc = q_con;
   f.dp = delpc;
@@ -437,28 +437,27 @@ int32_t fv3_riem_solver_c_f64(const fv3_domain* dom, double dt2, double ptop, do
   f.surf = hs;
   f.wsd = ws;
   f.z = gz;
-  f.w = const_cast<double*>(w3);  // read only in this solver
+  f.w = const_cast<S*>(w3);  // read only in this solver
   f.pef = pef;
   f.work = work;
   f.slab = work_slab;
   dim3 grid((unsigned)(((int64_t)bx.nib * bx.njb + RW - 1) / RW), d.ntiles);
   // cappa, ptc, q_con, delpc, gz, w3 read; gz, pef written (per cell-level)
   TimedLaunch timed("riem_c_kernel", as_stream(stream),
-                    8.0 * sizeof(double) * bx.nib * bx.njb * (double)dom->nk * d.ntiles);
-  hipLaunchKernelGGL(riem_kernel<true>, grid, dim3(RW), riem_lds_bytes(dom->nk),
+                    (4.0 * sizeof(S) + 4.0 * sizeof(double)) * bx.nib * bx.njb *
+                        (double)dom->nk * d.ntiles);
+  hipLaunchKernelGGL((riem_kernel<true, S>), grid, dim3(RW), riem_lds_bytes(dom->nk),
                      as_stream(stream), d, bx, o, dt2, p_fac, f);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
 }
 
-int32_t fv3_riem_solver3_f64(const fv3_domain* dom, int32_t last_call, double dt, double ptop,
-                             double p_fac, int32_t use_logp, double beta, const double* cappa,
-                             const double* zs, const double* wsd, double* delz,
-                             const double* q_con, const double* delp, const double* pt,
-                             double* zh, double* pe, double* ppe, double* pk3, double* pk,
-                             double* peln, double* w, double* work, int64_t work_slab,
-                             void* stream) {
-  using namespace fv3;
+template <typename S>
+int riem_solver3(const fv3_domain* dom, int last_call, double dt, double ptop, double p_fac,
+                 int use_logp, double beta, const S* cappa, const double* zs, const double* wsd,
+                 double* delz, const S* q_con, const S* delp, const S* pt, double* zh, double* pe,
+                 double* ppe, double* pk3, double* pk, double* peln, S* w, double* work,
+                 int64_t work_slab, void* stream) {
   const char* who = "fv3_riem_solver3";
   if (!dom) {
     set_error("%s: domain is NULL", who);
@@ -485,7 +484,7 @@ int32_t fv3_riem_solver3_f64(const fv3_domain* dom, int32_t last_call, double dt
   o.ptk = exp(o.kappa * o.peln1);
   o.rgrav = 1.0 / kGrav;
   Dom d = to_dom(dom);
-  RiemFields f{};
+  RiemFields<S> f{};
   f.cappa = cappa;
   f.qc = q_con;
   f.dp = delp;
@@ -506,11 +505,56 @@ int32_t fv3_riem_solver3_f64(const fv3_domain* dom, int32_t last_call, double dt
   // cappa, q_con, delp, pt, zh, w read; zh, ppe, pk3, delz, w written, plus
   // pe, pk and peln on the last call (per cell-level)
   TimedLaunch timed("riem3_kernel", as_stream(stream),
-                    (11.0 + 3.0 * o.last_call) * sizeof(double) * bx.nib * bx.njb *
+                    ((5.0 * sizeof(S) + (6.0 + 3.0 * o.last_call) * sizeof(double))) * bx.nib * bx.njb *
                         (double)dom->nk * d.ntiles);
-  hipLaunchKernelGGL(riem_kernel<false>, grid, dim3(RW), riem_lds_bytes(dom->nk),
+  hipLaunchKernelGGL((riem_kernel<false, S>), grid, dim3(RW), riem_lds_bytes(dom->nk),
                      as_stream(stream), d, bx, o, dt, p_fac, f);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
+}
+}  // namespace fv3
+
+extern "C" {
+
+int32_t fv3_riem_solver_c_f64(const fv3_domain* dom, double dt2, double ptop, double p_fac,
+                              const double* cappa, const double* hs, const double* ws,
+                              const double* ptc, const double* q_con, const double* delpc,
+                              double* gz, double* pef, const double* w3, double* work,
+                              int64_t work_slab, void* stream) {
+  return fv3::riem_solver_c<double>(dom, dt2, ptop, p_fac, cappa, hs, ws, ptc, q_con, delpc, gz,
+                                    pef, w3, work, work_slab, stream);
+}
+
+int32_t fv3_riem_solver_c_mixed(const fv3_domain* dom, double dt2, double ptop, double p_fac,
+                                const float* cappa, const double* hs, const double* ws,
+                                const float* ptc, const float* q_con, const float* delpc,
+                                double* gz, double* pef, const float* w3, double* work,
+                                int64_t work_slab, void* stream) {
+  return fv3::riem_solver_c<float>(dom, dt2, ptop, p_fac, cappa, hs, ws, ptc, q_con, delpc, gz,
+                                   pef, w3, work, work_slab, stream);
+}
+
+int32_t fv3_riem_solver3_f64(const fv3_domain* dom, int32_t last_call, double dt, double ptop,
+                             double p_fac, int32_t use_logp, double beta, const double* cappa,
+                             const double* zs, const double* wsd, double* delz,
+                             const double* q_con, const double* delp, const double* pt,
+                             double* zh, double* pe, double* ppe, double* pk3, double* pk,
+                             double* peln, double* w, double* work, int64_t work_slab,
+                             void* stream) {
+  return fv3::riem_solver3<double>(dom, last_call, dt, ptop, p_fac, use_logp, beta, cappa, zs,
+                                   wsd, delz, q_con, delp, pt, zh, pe, ppe, pk3, pk, peln, w,
+                                   work, work_slab, stream);
+}
+
+int32_t fv3_riem_solver3_mixed(const fv3_domain* dom, int32_t last_call, double dt, double ptop,
+                               double p_fac, int32_t use_logp, double beta, const float* cappa,
+                               const double* zs, const double* wsd, double* delz,
+                               const float* q_con, const float* delp, const float* pt,
+                               double* zh, double* pe, double* ppe, double* pk3, double* pk,
+                               double* peln, float* w, double* work, int64_t work_slab,
+                               void* stream) {
+  return fv3::riem_solver3<float>(dom, last_call, dt, ptop, p_fac, use_logp, beta, cappa, zs,
+                                  wsd, delz, q_con, delp, pt, zh, pe, ppe, pk3, pk, peln, w,
+                                  work, work_slab, stream);
 }
 }

@@ -45,7 +45,8 @@ def replicate(t: torch.Tensor, members: int) -> torch.Tensor:
 def ensemble_acoustic(init: Dict, members: int, cfg: DynamicalCoreConfig, seed: int = 0,
                       pt_sd: float = 0.5, uv_sd: float = 1.0, use_graph: bool = False,
                       mixed_precision: bool = False):
-    """(AcousticDynamics, state) for ``members`` perturbed copies of a cube state.
+    """(AcousticDynamics, state) for ``members`` perturbed copies of a cube state
+    (with ``mixed_precision``, the mixed form and its state: bulk fields FP32).
 
     ``init`` is what :func:`fv3core_amd.workloads.cube_state` returns.  Member
     m gets pt multiplied by (1 + N(0, pt_sd) / 288) (about pt_sd kelvin near
@@ -89,6 +90,10 @@ def ensemble_acoustic(init: Dict, members: int, cfg: DynamicalCoreConfig, seed: 
     acoustic = AcousticDynamics(comm, factory, grid, damp, 0, False, False, cfg.acoustic_dynamics,
                                 init["ak"], init["bk"], init_pfull(init["ak"], init["bk"], 1.0e5),
                                 phis, use_graph=use_graph, mixed_precision=mixed_precision)
+    if mixed_precision:
+        from .mixed import mixed_state
+
+        state = mixed_state(state)
     return acoustic, state
 
 

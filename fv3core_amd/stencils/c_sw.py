@@ -53,17 +53,20 @@ class CGridShallowWaterDynamics:
     def __call__(self, delp, pt, u, v, w, uc, vc, ua, va, ut, vt, divgd, omga, dt2: float):
         """Returns (delpc, ptc); uc, vc, ua, va, ut, vt, divgd, omga in place."""
         _lib.require_device(delp, pt, u, v, w, uc, vc, ua, va, ut, vt, divgd, omga)
-        if delp.dtype != torch.float64:
-            raise TypeError("c_sw runs in FP64: pass float64 storages")
+        if delp.dtype not in (torch.float64, torch.float32):
+            raise TypeError("c_sw runs on float64 (or, in the mixed form, float32) storages")
+        if self.grid_data.dx.dtype != delp.dtype:
+            raise TypeError("c_sw: the grid data and the storages must share one dtype")
+        scalar = ctypes.c_double if delp.dtype == torch.float64 else ctypes.c_float
         met, ref2 = _lib.metrics_struct(self.grid_data, delp.dtype, C_SW_METRICS)
         d = _lib.make_domain(self._grid_indexing, delp, self._nk, ref2,
                              others3=(pt, u, v, w, uc, vc, ua, va, ut, vt, divgd, omga,
                                       self.delpc, self.ptc))
         work, slab = self._work.get(delp, self._nk)
         _lib.check(
-            _lib.lib().fv3_c_sw_f64(
+            _lib.fn("fv3_c_sw", delp.dtype)(
                 ctypes.byref(d), ctypes.byref(met), ctypes.c_int32(self._nord),
-                ctypes.c_double(dt2), _lib.ptr(delp), _lib.ptr(pt), _lib.ptr(u), _lib.ptr(v),
+                scalar(dt2), _lib.ptr(delp), _lib.ptr(pt), _lib.ptr(u), _lib.ptr(v),
                 _lib.ptr(w), _lib.ptr(uc), _lib.ptr(vc), _lib.ptr(ua), _lib.ptr(va),
                 _lib.ptr(ut), _lib.ptr(vt), _lib.ptr(divgd), _lib.ptr(omga),
                 _lib.ptr(self.delpc), _lib.ptr(self.ptc), _lib.ptr(work), ctypes.c_int64(slab),

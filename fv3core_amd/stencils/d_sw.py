@@ -175,10 +175,10 @@ class DGridShallowWaterLagrangianDynamics:
         # mixed precision: FP32 arithmetic in the fv_tp_2d calls
         self.transport_fp32 = False
 
-    def _columns(self, device):
-        key = str(device)
+    def _columns(self, device, dtype):
+        key = (str(device), dtype)
         if key not in self._dev_cols:
-            self._dev_cols[key] = {k: torch.as_tensor(np.ascontiguousarray(v), dtype=torch.float64,
+            self._dev_cols[key] = {k: torch.as_tensor(np.ascontiguousarray(v), dtype=dtype,
                                                       device=device)
                                    for k, v in self._cols.items()}
         return self._dev_cols[key]
@@ -190,26 +190,32 @@ class DGridShallowWaterLagrangianDynamics:
                       ua=ua, va=va, divgd=divgd, mfx=mfx, mfy=mfy, cx=cx, cy=cy, crx=crx,
                       cry=cry, xfx=xfx, yfx=yfx, q_con=q_con, zh=zh, heat_source=heat_source,
                       diss_est=diss_est)
-        _lib.require_device(*fields.values())
-        if delp.dtype != torch.float64:
-            raise TypeError("d_sw runs in FP64: pass float64 storages")
+        if delp.dtype == torch.float32 and zh is not None and zh.dtype != torch.float32:
+            # the FP32 form does not read zh (the mixed step keeps it FP64)
+            fields["zh"] = None
+        present = [f for f in fields.values() if f is not None]
+        _lib.require_device(*present)
+        if delp.dtype not in (torch.float64, torch.float32):
+            raise TypeError("d_sw runs on float64 (or, in the mixed form, float32) storages")
+        if self.grid.dx.dtype != delp.dtype:
+            raise TypeError("d_sw: the grid data and the storages must share one dtype")
         met, ref2 = _lib.metrics_struct(self.grid, delp.dtype, D_SW_METRICS)
-        d = _lib.make_domain(self._gi, delp, self._nk + 1, ref2,
-                             others3=[f for f in fields.values()])
+        d = _lib.make_domain(self._gi, delp, self._nk + 1, ref2, others3=present)
         d.nk = self._nk
-        cols = self._columns(delp.device)
+        cols = self._columns(delp.device, delp.dtype)
         p = self._params
         p.transport_fp32 = int(bool(self.transport_fp32))
         for k, t in cols.items():
             setattr(p, k, t.data_ptr())
         st = _State()
         for k, t in fields.items():
-            setattr(st, k, t.data_ptr())
+            setattr(st, k, t.data_ptr() if t is not None else None)
         work, slab = self._work.get(delp, self._nk + 1)
+        scalar = ctypes.c_double if delp.dtype == torch.float64 else ctypes.c_float
         _lib.check(
-            _lib.lib().fv3_d_sw_f64(
+            _lib.fn("fv3_d_sw", delp.dtype)(
                 ctypes.byref(d), ctypes.byref(met), ctypes.byref(p), ctypes.byref(st),
-                ctypes.c_double(dt), _lib.ptr(work), ctypes.c_int64(slab),
+                scalar(dt), _lib.ptr(work), ctypes.c_int64(slab),
                 ctypes.c_void_p(_lib.stream_handle()),
             )
         )

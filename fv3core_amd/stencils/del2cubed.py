@@ -3,6 +3,8 @@ from __future__ import annotations
 
 import ctypes
 
+import torch
+
 from .. import _lib
 from ..utils.grid import DampingCoefficients
 from ..utils.stencil import StencilFactory
@@ -27,7 +29,8 @@ class HyperdiffusionDamping:
         d = _lib.make_domain(self._gi, qdel, nk, self._rarea,
                              others2=(self._del6_u, self._del6_v))
         work, slab = self._work.get(qdel, nk)
-        _lib.check(_lib.lib().fv3_del2cubed_f64(
+        scalar = ctypes.c_float if qdel.dtype == torch.float32 else ctypes.c_double
+        _lib.check(_lib.fn("fv3_del2cubed", qdel.dtype)(
             ctypes.byref(d), _lib.ptr(self._del6_u), _lib.ptr(self._del6_v), _lib.ptr(self._rarea),
-            ctypes.c_int32(self._ntimes), ctypes.c_double(cd), _lib.ptr(qdel), _lib.ptr(work),
+            ctypes.c_int32(self._ntimes), scalar(cd), _lib.ptr(qdel), _lib.ptr(work),
             ctypes.c_int64(slab), ctypes.c_void_p(_lib.stream_handle())))

@@ -8,6 +8,14 @@ inside a call.  The state is a namespace of storages as in the reference;
 the component's temporaries are added to it on the first call, as
 ``state.__dict__.update(self._temporaries)`` does there.
 
+With ``mixed_precision=True`` the step runs the mixed form of
+fv3core_amd.mixed: the state's bulk fields (``mixed.BULK_FIELDS``) are FP32
+storages advanced by the FP32 instantiations of c_sw, d_sw, Rayleigh
+damping, del2cubed and the halo exchange on FP32 metric terms, while the
+Riemann solvers, the height updates, the pressure gradients and the pk / pe
+halos run in FP64 on FP64 heights and pressures (reading the FP32 fields
+directly or through FP64 copies).
+
 With ``use_graph=True`` the whole call (every substep, halo exchange and
 component) is captured once into a HIP graph and replayed on later calls
 that pass the same storages and scalars; the first call for a new set runs
@@ -22,6 +30,7 @@ import numpy as np
 import torch
 
 from .. import _lib
+from .. import mixed as _mixed
 from .._config import AcousticDynamicsConfig
 from ..constants import GRAV, KAPPA
 from ..utils import gt4py_utils as utils
@@ -96,11 +105,26 @@ class AcousticDynamics:
         dev, dt_ = grid_data.dx.device, grid_data.dx.dtype
         shape3 = lead + tuple(gi.max_shape)
         shape2 = lead + tuple(gi.max_shape[:2])
+        self._mixed = bool(mixed_precision)
+        # storage type of the bulk fields (FP32 in the mixed form)
+        bulk = torch.float32 if self._mixed else dt_
         self._temporaries: Dict[str, torch.Tensor] = {}
-        for name in ("ut", "vt", "gz", "zh", "pem", "pkc", "pk3", "heat_source", "divgd",
-                     "crx", "xfx", "cry", "yfx"):
+        for name in ("ut", "vt", "heat_source", "divgd", "crx", "xfx", "cry", "yfx"):
+            self._temporaries[name] = utils.zeros(shape3, dtype=bulk, device=dev)
+        for name in ("gz", "zh", "pem", "pkc", "pk3"):
             self._temporaries[name] = utils.zeros(shape3, dtype=dt_, device=dev)
         self._temporaries["ws3"] = utils.zeros(shape2, dtype=dt_, device=dev, batch_dims=len(lead))
+        # FP64 copies of FP32 fields for the FP64 kernels that read them
+        # through a copy (height updates, pressure halos, nh_p_grad's delp)
+        self._f64 = {}
+        if self._mixed:
+            for name in ("ut", "vt", "crx", "cry", "xfx", "yfx", "delp"):
+                self._f64[name] = utils.zeros(shape3, dtype=dt_, device=dev)
+        grid_bulk = _mixed.grid_fp32(grid_data) if self._mixed else grid_data
+        damp_bulk = (DampingCoefficients(grid_bulk.del6_u, grid_bulk.del6_v,
+                                         damping_coefficients.da_min,
+                                         damping_coefficients.da_min_c)
+                     if self._mixed else damping_coefficients)
         self._temporaries["gz"].fill_(HUGE_R)
         if not config.hydrostatic:
             self._temporaries["pk3"].fill_(HUGE_R)
@@ -127,20 +151,16 @@ class AcousticDynamics:
             self.riem_solver3 = RiemannSolver3(stencil_factory, config.riemann)
             self.riem_solver_c = RiemannSolverC(stencil_factory, p_fac=config.p_fac)
         self.dgrid_shallow_water_lagrangian_dynamics = DGridShallowWaterLagrangianDynamics(
-            stencil_factory, grid_data, damping_coefficients, column_namelist, nested,
-            stretched_grid, dsw_cfg)
-        # mixed precision: FP32 arithmetic in d_sw's
-        # fv_tp_2d transports; storages and every other kernel stay FP64
-        self.dgrid_shallow_water_lagrangian_dynamics.transport_fp32 = bool(mixed_precision)
+            stencil_factory, grid_bulk, damp_bulk, column_namelist, nested, stretched_grid,
+            dsw_cfg)
         self.cgrid_shallow_water_lagrangian_dynamics = CGridShallowWaterDynamics(
-            stencil_factory, grid_data, nested, config.grid_type, config.nord)
+            stencil_factory, grid_bulk, nested, config.grid_type, config.nord)
         self.update_geopotential_height_on_c_grid = UpdateGeopotentialHeightOnCGrid(
             stencil_factory, grid_data.area)
         self._do_del2cubed = self._nk_heat_dissipation != 0 and config.d_con > 1.0e-5
         if self._do_del2cubed:
             self._hyperdiffusion = HyperdiffusionDamping(
-                stencil_factory, damping_coefficients, grid_data.rarea,
-                nmax=min(3, config.nord + 1))
+                stencil_factory, damp_bulk, grid_bulk.rarea, nmax=min(3, config.nord + 1))
         if config.rf_fast:
             self._rayleigh_damping = RayleighDamping(stencil_factory, rf_cutoff=config.rf_cutoff,
                                                      tau=config.tau,
@@ -157,7 +177,7 @@ class AcousticDynamics:
     # ------------------------------------------------------------ helpers
     def _column(self, op, nk, out, a=None, b=None, scalar=0.0):
         d = _lib.make_domain(self._gi, out, nk)
-        _lib.check(_lib.lib().fv3_dyncore_column_f64(
+        _lib.check(_lib.fn("fv3_dyncore_column", out.dtype)(
             ctypes.byref(d), ctypes.c_int32(op), ctypes.c_double(scalar), _lib.ptr(a),
             _lib.ptr(b), _lib.ptr(out), ctypes.c_void_p(_lib.stream_handle())))
 
@@ -174,7 +194,8 @@ class AcousticDynamics:
         g = self.grid_data
         d = _lib.make_domain(self._gi, uc, self._npz, g.rdxc, others3=(vc, delpc, pkc, gz),
                              others2=(g.rdyc,))
-        _lib.check(_lib.lib().fv3_p_grad_c_f64(
+        name = "fv3_p_grad_c_mixed" if uc.dtype == torch.float32 else "fv3_p_grad_c_f64"
+        _lib.check(getattr(_lib.lib(), name)(
             ctypes.byref(d), _lib.ptr(g.rdxc), _lib.ptr(g.rdyc),
             ctypes.c_int32(int(bool(self.config.hydrostatic))), _lib.ptr(uc), _lib.ptr(vc),
             _lib.ptr(delpc), _lib.ptr(pkc), _lib.ptr(gz), ctypes.c_double(dt2),
@@ -221,6 +242,15 @@ class AcousticDynamics:
         dt2 = 0.5 * dt
         n_split = config.n_split
         hydro = config.hydrostatic
+        mixed = self._mixed
+        if mixed:
+            _mixed.check_mixed_state(state)
+        f64 = self._f64
+
+        def as64(name, t):
+            """t itself (FP64 form) or its FP64 copy (mixed form)."""
+            return _mixed.cast_into(t, f64[name]) if mixed else t
+
         self._h_scalar.update([state.q_con, state.cappa])
         self._h_scalar.update([state.delp, state.pt])
         self._h_vector_d.update([state.u], [state.v])
@@ -234,7 +264,8 @@ class AcousticDynamics:
                     self._column(_OP_SET_GZ, npz + 1, state.gz, a=self._zs, b=state.delz)
                     self._h_scalar.update([state.gz])
             if it == n_split - 1 and end_step and config.use_old_omega:
-                self._column(_OP_SET_PEM, npz, state.pem, a=state.delp, scalar=state.ptop)
+                self._column(_OP_SET_PEM, npz, state.pem, a=as64("delp", state.delp),
+                             scalar=state.ptop)
             state.delpc, state.ptc = self.cgrid_shallow_water_lagrangian_dynamics(
                 state.delp, state.pt, state.u, state.v, state.w, state.uc, state.vc, state.ua,
                 state.va, state.ut, state.vt, state.divgd, state.omga, dt2)
@@ -246,7 +277,8 @@ class AcousticDynamics:
                 else:
                     self._column(_OP_COPY, npz + 1, state.gz, a=state.zh)
                 self.update_geopotential_height_on_c_grid(
-                    self._dp_ref, self._zs, state.ut, state.vt, state.gz, state.ws3, dt2)
+                    self._dp_ref, self._zs, as64("ut", state.ut), as64("vt", state.vt), state.gz,
+                    state.ws3, dt2)
                 self.riem_solver_c(dt2, state.cappa, state.ptop, state.phis, state.ws3,
                                    state.ptc, state.q_con, state.delpc, state.gz, state.pkc,
                                    state.omga)
@@ -259,21 +291,22 @@ class AcousticDynamics:
                 state.heat_source, state.diss_estd, dt)
             self._h_scalar.update([state.delp, state.pt, state.q_con])
             if not hydro:
-                self.update_height_on_d_grid(self._zs, state.zh, state.crx, state.cry,
-                                             state.xfx, state.yfx, state.wsd, dt)
+                self.update_height_on_d_grid(self._zs, state.zh, as64("crx", state.crx),
+                                             as64("cry", state.cry), as64("xfx", state.xfx),
+                                             as64("yfx", state.yfx), state.wsd, dt)
                 self.riem_solver3(remap_step, dt, state.cappa, state.ptop, self._zs, state.wsd,
                                   state.delz, state.q_con, state.delp, state.pt, state.zh,
                                   state.pe, state.pkc, state.pk3, state.pk, state.peln,
                                   state.w)
                 self._h_scalar.update([state.zh])
                 self._h_pkc.update([state.pkc])
+                delp64 = as64("delp", state.delp)
                 if remap_step:
-                    edge_pe(self._gi, state.pe, state.delp, state.ptop)
-                self._pk3_halo(state.pk3, state.delp, state.ptop, akap)
+                    edge_pe(self._gi, state.pe, delp64, state.ptop)
+                self._pk3_halo(state.pk3, delp64, state.ptop, akap)
                 self._column(_OP_GEOPOTENTIAL, npz + 1, state.gz, a=state.zh)
                 self.nonhydrostatic_pressure_gradient(state.u, state.v, state.pkc, state.gz,
-                                                      state.pk3, state.delp, dt, state.ptop,
-                                                      akap)
+                                                      state.pk3, delp64, dt, state.ptop, akap)
             if config.rf_fast:
                 self._rayleigh_damping(state.u, state.v, state.w, self._dp_ref_host[:npz],
                                        self._pfull, dt, state.ptop, state.ks)

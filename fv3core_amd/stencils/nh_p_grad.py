@@ -8,6 +8,8 @@ from __future__ import annotations
 
 import ctypes
 
+import torch
+
 from .. import _lib
 from ..utils.grid import GridData
 from ..utils.stencil import StencilFactory
@@ -29,14 +31,21 @@ class NonHydrostaticPressureGradient:
     def __call__(self, u, v, pp, gz, pk3, delp, dt: float, ptop: float, akap: float):
         """u, v updated; pp, gz, pk3 replaced on the B grid (as the reference does)."""
         _lib.require_device(u, v, pp, gz, pk3, delp)
-        met, ref2 = _lib.metrics_struct(self._grid, u.dtype, A2B_METRICS + ("rdx", "rdy"))
+        # u and v are FP64, or FP32 in the mixed form; the rest is FP64
+        for t in (pp, gz, pk3, delp):
+            if t.dtype != torch.float64:
+                raise TypeError("nh_p_grad takes pp, gz, pk3 and delp in FP64")
+        if u.dtype != v.dtype or u.dtype not in (torch.float64, torch.float32):
+            raise TypeError("u and v must share one dtype (float64 or float32)")
+        name = "fv3_nh_p_grad_f64" if u.dtype == torch.float64 else "fv3_nh_p_grad_mixed"
+        met, ref2 = _lib.metrics_struct(self._grid, torch.float64, A2B_METRICS + ("rdx", "rdy"))
         d = _lib.make_domain(self._gi, pp, self.nk + 1, ref2, others3=(gz, pk3))
         d.nk = self.nk
         _lib.make_domain(self._gi, u, self.nk, ref2, others3=(v, delp, pp))
         ptk = ptop ** akap
         work, slab = self._work.get(pp, self.nk + 1)
         _lib.check(
-            _lib.lib().fv3_nh_p_grad_f64(
+            getattr(_lib.lib(), name)(
                 ctypes.byref(d), ctypes.byref(met), _lib.ptr(u), _lib.ptr(v), _lib.ptr(pp),
                 _lib.ptr(gz), _lib.ptr(pk3), _lib.ptr(delp), ctypes.c_double(dt),
                 ctypes.c_double(ptk), _lib.ptr(work), ctypes.c_int64(slab),

@@ -62,7 +62,7 @@ class RayleighDamping:
     def __call__(self, u, v, w, dp, pfull, dt: float, ptop: float, ks: int):
         _lib.require_device(u, v, w)
         nz = self._gi.domain[2]
-        key = (float(dt), float(ptop), str(u.device))
+        key = (float(dt), float(ptop), str(u.device), u.dtype)
         if key not in self._cache:
             dp_h = dp.detach().cpu().numpy() if isinstance(dp, torch.Tensor) else np.asarray(dp)
             pf_h = (pfull.detach().cpu().numpy() if isinstance(pfull, torch.Tensor)
@@ -71,15 +71,16 @@ class RayleighDamping:
                                                      self._rf_cutoff, self._tau)
             dev = u.device
             self._cache[key] = (
-                torch.as_tensor(rf, dtype=torch.float64, device=dev),
-                torch.as_tensor(np.asarray(dp_h[:nz], dtype=np.float64), device=dev),
-                torch.as_tensor(dm, dtype=torch.float64, device=dev),
+                torch.as_tensor(rf, dtype=u.dtype, device=dev),
+                torch.as_tensor(np.asarray(dp_h[:nz], dtype=np.float64), dtype=u.dtype,
+                                device=dev),
+                torch.as_tensor(dm, dtype=u.dtype, device=dev),
                 torch.as_tensor(use_rf.astype(np.int32), device=dev),
                 torch.as_tensor(use_n.astype(np.int32), device=dev),
             )
         rf, dpd, dm, urf, un = self._cache[key]
         d = _lib.make_domain(self._gi, u, nz, others3=(v, w))
-        _lib.check(_lib.lib().fv3_ray_fast_f64(
+        _lib.check(_lib.fn("fv3_ray_fast", u.dtype)(
             ctypes.byref(d), ctypes.c_int32(int(self._hydrostatic)), _lib.ptr(rf), _lib.ptr(dpd),
             _lib.ptr(dm), _lib.ptr(urf), _lib.ptr(un), _lib.ptr(u), _lib.ptr(v), _lib.ptr(w),
             ctypes.c_void_p(_lib.stream_handle())))

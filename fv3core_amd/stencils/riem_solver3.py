@@ -30,15 +30,21 @@ class RiemannSolver3:
                  delp, pt, zh, pe, ppe, pk3, pk, peln, w):
         fields = (cappa, delz, q_con, delp, pt, zh, pe, ppe, pk3, pk, peln, w)
         _lib.require_device(zs, wsd, *fields)
-        for t in fields:
+        for t in (delz, zh, pe, ppe, pk3, pk, peln):
             if t.dtype != torch.float64:
-                raise TypeError("the Riemann solvers run in FP64: pass float64 storages")
+                raise TypeError("the Riemann solvers integrate the heights and pressures in FP64")
+        # cappa, q_con, delp, pt and w are FP64, or FP32 in the mixed form
+        bulk = cappa.dtype
+        if bulk not in (torch.float64, torch.float32) or any(
+                t.dtype != bulk for t in (q_con, delp, pt, w)):
+            raise TypeError("cappa, q_con, delp, pt and w must share one dtype")
+        name = "fv3_riem_solver3_f64" if bulk == torch.float64 else "fv3_riem_solver3_mixed"
         d = _lib.make_domain(self._grid_indexing, zh, self._nk, zs,
                              others3=fields, others2=(wsd,))
         work, slab = self._work.get(zh, self._nk)
         c = self._config
         _lib.check(
-            _lib.lib().fv3_riem_solver3_f64(
+            getattr(_lib.lib(), name)(
                 ctypes.byref(d), int(bool(last_call)), ctypes.c_double(dt),
                 ctypes.c_double(ptop), ctypes.c_double(c.p_fac), int(bool(c.use_logp)),
                 ctypes.c_double(c.beta), _lib.ptr(cappa), _lib.ptr(zs), _lib.ptr(wsd),

@@ -26,14 +26,20 @@ class RiemannSolverC:
     def __call__(self, dt2: float, cappa, ptop: float, hs, ws, ptc, q_con, delpc, gz, pef, w3):
         """gz, pef <- nonhydrostatic C-grid solve (w3 is read only)."""
         _lib.require_device(cappa, hs, ws, ptc, q_con, delpc, gz, pef, w3)
-        for t in (cappa, ptc, q_con, delpc, gz, pef, w3):
+        for t in (gz, pef):
             if t.dtype != torch.float64:
-                raise TypeError("the Riemann solvers run in FP64: pass float64 storages")
+                raise TypeError("the Riemann solvers integrate gz and pef in FP64")
+        # cappa, ptc, q_con, delpc and w3 are FP64, or FP32 in the mixed form
+        bulk = cappa.dtype
+        if bulk not in (torch.float64, torch.float32) or any(
+                t.dtype != bulk for t in (ptc, q_con, delpc, w3)):
+            raise TypeError("cappa, ptc, q_con, delpc and w3 must share one dtype")
+        name = "fv3_riem_solver_c_f64" if bulk == torch.float64 else "fv3_riem_solver_c_mixed"
         d = _lib.make_domain(self._grid_indexing, gz, self._nk, hs,
                              others3=(cappa, ptc, q_con, delpc, pef, w3), others2=(ws,))
         work, slab = self._work.get(gz, self._nk)
         _lib.check(
-            _lib.lib().fv3_riem_solver_c_f64(
+            getattr(_lib.lib(), name)(
                 ctypes.byref(d), ctypes.c_double(dt2), ctypes.c_double(ptop),
                 ctypes.c_double(self._p_fac), _lib.ptr(cappa), _lib.ptr(hs), _lib.ptr(ws),
                 _lib.ptr(ptc), _lib.ptr(q_con), _lib.ptr(delpc), _lib.ptr(gz), _lib.ptr(pef),

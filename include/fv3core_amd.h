@@ -395,6 +395,13 @@ int32_t fv3_c_sw_f64(const fv3_domain* d, const fv3_metrics* g, int32_t nord, do
                      double* uc, double* vc, double* ua, double* va, double* ut, double* vt,
                      double* divgd, double* omga, double* delpc, double* ptc, double* work,
                      int64_t work_slab, void* stream);
+/* The same with FP32 storages and metric terms, in FP32 arithmetic (the bulk
+ * stencils of the mixed-precision acoustic step). */
+int32_t fv3_c_sw_f32(const fv3_domain* d, const fv3_metrics* g, int32_t nord, float dt2,
+                     float* delp, float* pt, const float* u, const float* v, float* w, float* uc,
+                     float* vc, float* ua, float* va, float* ut, float* vt, float* divgd,
+                     float* omga, float* delpc, float* ptc, float* work, int64_t work_slab,
+                     void* stream);
 
 /*
  * AGrid2BGridFourthOrder.__call__ (fv3core/stencils/a2b_ord4.py:617-695):
@@ -408,7 +415,9 @@ int32_t fv3_a2b_ord4_f64(const fv3_domain* d, const fv3_metrics* g, int32_t repl
 /*
  * Namelist values and per-level columns of DGridShallowWaterLagrangianDynamics
  * (fv3core/stencils/d_sw.py:606-842, columns from get_column_namelist
- * d_sw.py:512-563).  Column pointers are device arrays of d->nk entries.
+ * d_sw.py:512-563).  Column pointers are device arrays of d->nk entries in
+ * the storage type of the call: double for fv3_d_sw_f64, float for
+ * fv3_d_sw_f32.
  */
 typedef struct fv3_dsw_params {
   int32_t hord_dp, hord_tm, hord_mt, hord_vt;
@@ -421,15 +430,15 @@ typedef struct fv3_dsw_params {
   double dddmp, da_min_c;
   double dd8;                                  /* (da_min_c * d4_bg) ** (div_nord + 1) */
   double d_con;                                /* namelist d_con */
-  const double* damp_vt_deln;  /* DelnFlux damping of delp and pt (calc_damp, da_min) */
-  const double* damp_t_deln;   /* DelnFlux damping of q_con */
-  const double* damp_w_nosg;   /* DelnFluxNoSG damping of w (calc_damp, da_min_c) */
-  const double* damp_v_nosg;   /* DelnFluxNoSG damping of the vorticity */
-  const double* damp_w;        /* columns damp_w, damp_vt, ke_bg, d_con, d2_divg */
-  const double* damp_vt;
-  const double* ke_bg;
-  const double* d_con_col;
-  const double* d2_divg;
+  const void* damp_vt_deln;  /* DelnFlux damping of delp and pt (calc_damp, da_min) */
+  const void* damp_t_deln;   /* DelnFlux damping of q_con */
+  const void* damp_w_nosg;   /* DelnFluxNoSG damping of w (calc_damp, da_min_c) */
+  const void* damp_v_nosg;   /* DelnFluxNoSG damping of the vorticity */
+  const void* damp_w;        /* columns damp_w, damp_vt, ke_bg, d_con, d2_divg */
+  const void* damp_vt;
+  const void* ke_bg;
+  const void* d_con_col;
+  const void* d2_divg;
 } fv3_dsw_params;
 
 /* The storages d_sw reads and updates (same layout, nk + 1 levels each). */
@@ -437,6 +446,10 @@ typedef struct fv3_dsw_state {
   double *delpc, *delp, *ptc, *pt, *u, *v, *w, *uc, *vc, *ua, *va, *divgd;
   double *mfx, *mfy, *cx, *cy, *crx, *cry, *xfx, *yfx, *q_con, *zh, *heat_source, *diss_est;
 } fv3_dsw_state;
+typedef struct fv3_dsw_state_f32 {
+  float *delpc, *delp, *ptc, *pt, *u, *v, *w, *uc, *vc, *ua, *va, *divgd;
+  float *mfx, *mfy, *cx, *cy, *crx, *cry, *xfx, *yfx, *q_con, *zh, *heat_source, *diss_est;
+} fv3_dsw_state_f32;
 
 /*
  * DGridShallowWaterLagrangianDynamics.__call__ (fv3core/stencils/d_sw.py:843-1131):
@@ -447,6 +460,12 @@ typedef struct fv3_dsw_state {
  */
 int32_t fv3_d_sw_f64(const fv3_domain* d, const fv3_metrics* g, const fv3_dsw_params* p,
                      const fv3_dsw_state* st, double dt, double* work, int64_t work_slab,
+                     void* stream);
+/* The same with FP32 storages, metric terms and columns, in FP32 arithmetic
+ * (the bulk stencils of the mixed-precision acoustic step); zh may be NULL
+ * (d_sw does not read it); transport_fp32 is ignored. */
+int32_t fv3_d_sw_f32(const fv3_domain* d, const fv3_metrics* g, const fv3_dsw_params* p,
+                     const fv3_dsw_state_f32* st, float dt, float* work, int64_t work_slab,
                      void* stream);
 
 /*
@@ -484,6 +503,11 @@ int32_t fv3_update_dz_d_f64(const fv3_domain* d, const fv3_metrics* g, int32_t h
 int32_t fv3_nh_p_grad_f64(const fv3_domain* d, const fv3_metrics* g, double* u, double* v,
                           double* pp, double* gz, double* pk3, const double* delp, double dt,
                           double ptk, double* work, int64_t work_slab, void* stream);
+/* Mixed-precision form (no counterpart in the reference, which is FP64
+ * only): u, v FP32 storages, everything else and the arithmetic FP64. */
+int32_t fv3_nh_p_grad_mixed(const fv3_domain* d, const fv3_metrics* g, float* u, float* v,
+                            double* pp, double* gz, double* pk3, const double* delp, double dt,
+                            double ptk, double* work, int64_t work_slab, void* stream);
 
 /* PK3Halo.__call__ (fv3core/stencils/pk3_halo.py:11-68): pk3 = pe ** akap on
  * the 2-wide ring around the compute domain, levels 1..d->nk-1. */
@@ -498,6 +522,11 @@ int32_t fv3_pe_halo_f64(const fv3_domain* d, double* pe, const double* delp, dou
 int32_t fv3_p_grad_c_f64(const fv3_domain* d, const double* rdxc, const double* rdyc,
                          int32_t hydrostatic, double* uc, double* vc, const double* delpc,
                          const double* pkc, const double* gz, double dt2, void* stream);
+/* Mixed-precision form: uc, vc, delpc FP32 storages; pkc, gz, the metric
+ * terms and the arithmetic FP64. */
+int32_t fv3_p_grad_c_mixed(const fv3_domain* d, const double* rdxc, const double* rdyc,
+                           int32_t hydrostatic, float* uc, float* vc, const float* delpc,
+                           const double* pkc, const double* gz, double dt2, void* stream);
 
 /*
  * RayleighDamping.__call__ (fv3core/stencils/ray_fast.py:130-192) with the
@@ -508,18 +537,37 @@ int32_t fv3_ray_fast_f64(const fv3_domain* d, int32_t hydrostatic, const double*
                          const double* dp, const double* dm, const int32_t* use_rf,
                          const int32_t* use_nudge, double* u, double* v, double* w,
                          void* stream);
+int32_t fv3_ray_fast_f32(const fv3_domain* d, int32_t hydrostatic, const float* rf,
+                         const float* dp, const float* dm, const int32_t* use_rf,
+                         const int32_t* use_nudge, float* u, float* v, float* w, void* stream);
 
 /* HyperdiffusionDamping.__call__ (fv3core/stencils/del2cubed.py:158-181):
  * min(3, nmax) del-2 passes on qdel with coefficient cd.  work: 2 slabs. */
 int32_t fv3_del2cubed_f64(const fv3_domain* d, const double* del6_u, const double* del6_v,
                           const double* rarea, int32_t nmax, double cd, double* qdel,
                           double* work, int64_t work_slab, void* stream);
+int32_t fv3_del2cubed_f32(const fv3_domain* d, const float* del6_u, const float* del6_v,
+                          const float* rarea, int32_t nmax, float cd, float* qdel, float* work,
+                          int64_t work_slab, void* stream);
 
 /* compute_pkz_tempadjust (fv3core/stencils/temperature_adjust.py:8-44) on
  * the compute domain, levels [0, d->nk). */
 int32_t fv3_pkz_tempadjust_f64(const fv3_domain* d, const double* delp, const double* delz,
                                const double* cappa, const double* heat_source, double* pt,
                                double* pkz, double delt_time_factor, void* stream);
+/* Mixed-precision form: delp, cappa, heat_source, pt FP32 storages; delz,
+ * pkz and the arithmetic FP64. */
+int32_t fv3_pkz_tempadjust_mixed(const fv3_domain* d, const float* delp, const double* delz,
+                                 const float* cappa, const float* heat_source, float* pt,
+                                 double* pkz, double delt_time_factor, void* stream);
+
+/*
+ * Element-wise conversion of n elements of one dense storage (no counterpart
+ * in the reference, which is FP64 only): the mixed-precision acoustic step
+ * hands its FP32 fields to the FP64 column solvers and back through these.
+ */
+int32_t fv3_cast_f32_to_f64(int64_t n, const float* src, double* dst, void* stream);
+int32_t fv3_cast_f64_to_f32(int64_t n, const double* src, float* dst, void* stream);
 
 /*
  * Small stencils of dyn_core (fv3core/stencils/dyn_core.py:47-98) by op:
@@ -530,6 +578,8 @@ int32_t fv3_pkz_tempadjust_f64(const fv3_domain* d, const double* delp, const do
  */
 int32_t fv3_dyncore_column_f64(const fv3_domain* d, int32_t op, double scalar, const double* a,
                                const double* b, double* out, void* stream);
+int32_t fv3_dyncore_column_f32(const fv3_domain* d, int32_t op, double scalar, const float* a,
+                               const float* b, float* out, void* stream);
 
 /* ---- vertical remapping (Lagrangian -> Eulerian) ---------------------- */
 
