@@ -6,8 +6,8 @@ path treats tiles independently while the halo update applies its six-tile
 map to each member, so one call of a component (AcousticDynamics, d_sw, the
 Riemann solvers, ...) advances every member with one launch per kernel.
 Members of different resolutions cannot share a storage; ``UnevenEnsemble``
-keeps one AcousticDynamics per resolution and runs them back to back
-(optionally each captured in a HIP graph).
+keeps one AcousticDynamics per resolution and runs them back to back,
+optionally all captured in one HIP graph launched once per call.
 
 The reference runs one cube per MPI job; ensembles are outside it.
 """
@@ -114,11 +114,43 @@ def member_slice(state, m: int):
 
 class UnevenEnsemble:
     """Members of several resolutions: one batched AcousticDynamics per
-    resolution, called in sequence by :meth:`__call__`."""
+    resolution, called in sequence by :meth:`__call__`.
 
-    def __init__(self, parts: Sequence):
+    With ``use_graph=True`` every resolution's call is captured into one HIP
+    graph, so that a call of the whole ensemble (all its acoustic substeps,
+    halo exchanges and components, for every batch) is one graph launch: the
+    first call runs eagerly (it fills the component caches), the second
+    captures and replays, later calls replay.  The parts' own graph option
+    must be off; the storages must stay the same between calls."""
+
+    def __init__(self, parts: Sequence, use_graph: bool = False):
         self.parts: List = list(parts)  # (acoustic, state) pairs
+        self._use_graph = bool(use_graph)
+        self._calls = 0
+        self.graph = None
 
-    def __call__(self):
+    def _run(self):
         for acoustic, state in self.parts:
             acoustic(state)
+
+    def __call__(self):
+        if not self._use_graph:
+            self._run()
+            return
+        if any(getattr(a, "_use_graph", False) for a, _ in self.parts):
+            raise ValueError("UnevenEnsemble(use_graph=True) captures the parts itself: "
+                             "build them with use_graph=False")
+        self._calls += 1
+        if self.graph is not None:
+            self.graph.replay()
+            return
+        if self._calls == 1:
+            self._run()
+            return
+        graph = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph):
+            self._run()
+        torch.cuda.synchronize()
+        self.graph = graph
+        graph.replay()

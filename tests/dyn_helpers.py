@@ -83,3 +83,60 @@ class Case:
                 t.copy_(torch.as_tensor(np.stack([m[k] for m in self.metric])))
             fields[k] = t
         return GridData(fields)
+
+
+def oracle_acoustic_call(init, cfg, n, nz, pfull):
+    """The oracle's AcousticDynamics call on the six tiles of one cube state
+    (what workloads.cube_state returns), with the host restatement of the
+    halo map between its components; returns the updated state dict."""
+    from fv3core_amd.halo import CENTER, CORNER, X_FACE, Y_FACE, apply_map_numpy, build_map, \
+        build_sync_map
+    from fv3core_amd.stencils.d_sw import get_column_namelist
+    from oracle import dyncore as odc
+
+    st0, m = init["state"], init["metric"]
+    S = {k: v.copy() for k, v in st0.items()}
+    S["phis"] = init["phis"].copy()
+    S["wsd"] = init["wsd"].copy()
+    S["ws3"] = np.zeros_like(init["wsd"])
+    shape = st0["delp"].shape
+    for k in ("ut", "vt", "gz", "pem", "pkc", "pk3", "heat_source", "divgd", "crx", "xfx",
+              "cry", "yfx", "delpc", "ptc"):
+        S[k] = np.zeros(shape)
+    S["gz"][:] = 1.0e40
+    S["pk3"][:] = 1.0e40
+    maps = {}
+
+    def halo(kind, names, n_halo=3):
+        key = (kind, n_halo)
+        if key not in maps:
+            stg = {"scalar": (CENTER,), "vector_d": (Y_FACE, X_FACE),
+                   "vector_c": (X_FACE, Y_FACE), "corner": (CORNER,)}[kind]
+            maps[key] = build_map(n, stg, n_halo, 3)
+        mp = maps[key]
+        if kind.startswith("vector"):
+            for a in range(0, len(names), 2):
+                u, v = apply_map_numpy([S[names[a]], S[names[a + 1]]], mp)
+                S[names[a]][...] = u
+                S[names[a + 1]][...] = v
+        else:
+            for name in names:
+                S[name][...] = apply_map_numpy([S[name]], mp)[0]
+
+    def sync():
+        mp = build_sync_map(n, 3)
+        u, v = apply_map_numpy([S["u"], S["v"]], mp)
+        S["u"][...] = u
+        S["v"][...] = v
+
+    col = get_column_namelist(cfg.d_grid_shallow_water, nz)
+    ocfg = dict(mdt=float(cfg.dt_atmos), n_split=cfg.n_split, k_split=cfg.k_split, n_map=1,
+                ptop=init["ptop"], hord_dp=cfg.hord_dp, hord_tm=cfg.hord_tm,
+                hord_mt=cfg.hord_mt, hord_vt=cfg.hord_vt, dddmp=cfg.dddmp, d4_bg=cfg.d4_bg,
+                nord=cfg.nord, d_con=cfg.d_con, convert_ke=cfg.convert_ke, vtdm4=cfg.vtdm4,
+                p_fac=cfg.p_fac, rf_cutoff=cfg.rf_cutoff, tau=cfg.tau, delt_max=cfg.delt_max)
+    metrics = [{k: v[t] for k, v in m.items() if isinstance(v, np.ndarray) and v.ndim == 3}
+               for t in range(6)]
+    odc.acoustic_dynamics(S, ocfg, metrics, dict(da_min=m["da_min"], da_min_c=m["da_min_c"]),
+                          init["ak"], init["bk"], pfull, halo, sync, col, nz)
+    return S

@@ -331,11 +331,14 @@ def test_gpu_remap_tlm_finite_differences(mode):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("hord", [8])
-def test_gpu_multistep_transport_remap_gradient(hord):
-    """8 steps of fv_tp_2d + map_scalar (kord 9): reverse sweep vs central FD of J."""
+@pytest.mark.parametrize("n", [24, 96])
+def test_gpu_multistep_transport_remap_gradient(hord, n):
+    """8 steps of fv_tp_2d + map_scalar (kord 9): reverse sweep vs central FD
+    of J, on a 24 x 24 tile and on the 96 x 96 x 79 tile of the adjoint
+    workload."""
     from fv3core_amd.adjoint import PeriodicTransport, cost_and_gradient
     from fv3core_amd.workloads import remap_inputs
-    n, nz, nsteps = 24, 79, 8
+    nz, nsteps = 79, 8
     d = _periodic(n, nz)
     r = remap_inputs(n, nz, workload=5)
     gi = GridIndexing((n, n, nz), N_HALO, False, False, False, False)
