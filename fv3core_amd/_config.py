@@ -99,6 +99,36 @@ class AcousticDynamicsConfig:
 
 
 @dataclasses.dataclass(frozen=True)
+class SatAdjustConfig:
+    """Namelist of the fast saturation adjustment (_config.py:86-110)."""
+
+    hydrostatic: bool = False
+    rad_snow: bool = True
+    rad_rain: bool = True
+    rad_graupel: bool = True
+    tintqs: bool = False
+    sat_adj0: float = 0.90
+    ql_gen: float = 1.0e-3
+    qs_mlt: float = 1.0e-6
+    ql0_max: float = 2.0e-3
+    t_sub: float = 184.0
+    qi_gen: float = 1.82e-6
+    qi_lim: float = 1.0
+    qi0_max: float = 1.0e-4
+    dw_ocean: float = 0.10
+    dw_land: float = 0.20
+    icloud_f: int = 0
+    cld_min: float = 0.05
+    tau_i2s: float = 1000.0
+    tau_v2l: float = 150.0
+    tau_r2g: float = 900.0
+    tau_l2r: float = 900.0
+    tau_l2v: float = 300.0
+    tau_imlt: float = 600.0
+    tau_smlt: float = 900.0
+
+
+@dataclasses.dataclass(frozen=True)
 class RemappingConfig:
     fill: bool
     kord_tm: int
@@ -106,6 +136,11 @@ class RemappingConfig:
     kord_wz: int
     kord_mt: int
     do_sat_adj: bool
+    sat_adjust: SatAdjustConfig = dataclasses.field(default_factory=SatAdjustConfig)
+
+    @property
+    def hydrostatic(self) -> bool:
+        return self.sat_adjust.hydrostatic
 
 
 @dataclasses.dataclass
@@ -169,6 +204,30 @@ class DynamicalCoreConfig:
     n_sponge: int = 48
     check_negative: bool = False
     c2l_ord: int = 4
+    # fast saturation adjustment (NamelistDefaults of the reference's _config.py)
+    tau_r2g: float = 900.0
+    tau_smlt: float = 900.0
+    tau_imlt: float = 600.0
+    tau_i2s: float = 1000.0
+    tau_l2r: float = 900.0
+    tau_l2v: float = 300.0
+    tau_v2l: float = 150.0
+    sat_adj0: float = 0.90
+    ql_gen: float = 1.0e-3
+    qs_mlt: float = 1.0e-6
+    ql0_max: float = 2.0e-3
+    t_sub: float = 184.0
+    qi_gen: float = 1.82e-6
+    qi_lim: float = 1.0
+    qi0_max: float = 1.0e-4
+    rad_snow: bool = True
+    rad_rain: bool = True
+    rad_graupel: bool = True
+    tintqs: bool = False
+    dw_ocean: float = 0.10
+    dw_land: float = 0.20
+    icloud_f: int = 0
+    cld_min: float = 0.05
 
     @property
     def riemann(self) -> RiemannConfig:
@@ -197,7 +256,19 @@ class DynamicalCoreConfig:
         )
 
     @property
+    def sat_adjust(self) -> SatAdjustConfig:
+        return SatAdjustConfig(
+            hydrostatic=self.hydrostatic, rad_snow=self.rad_snow, rad_rain=self.rad_rain,
+            rad_graupel=self.rad_graupel, tintqs=self.tintqs, sat_adj0=self.sat_adj0,
+            ql_gen=self.ql_gen, qs_mlt=self.qs_mlt, ql0_max=self.ql0_max, t_sub=self.t_sub,
+            qi_gen=self.qi_gen, qi_lim=self.qi_lim, qi0_max=self.qi0_max,
+            dw_ocean=self.dw_ocean, dw_land=self.dw_land, icloud_f=self.icloud_f,
+            cld_min=self.cld_min, tau_i2s=self.tau_i2s, tau_v2l=self.tau_v2l,
+            tau_r2g=self.tau_r2g, tau_l2r=self.tau_l2r, tau_l2v=self.tau_l2v,
+            tau_imlt=self.tau_imlt, tau_smlt=self.tau_smlt)
+
+    @property
     def remapping(self) -> RemappingConfig:
         return RemappingConfig(fill=self.fill, kord_tm=self.kord_tm, kord_tr=self.kord_tr,
                                kord_wz=self.kord_wz, kord_mt=self.kord_mt,
-                               do_sat_adj=self.do_sat_adj)
+                               do_sat_adj=self.do_sat_adj, sat_adjust=self.sat_adjust)

@@ -236,6 +236,30 @@ __global__ void __launch_bounds__(256) remap_prep_kernel(Dom d, RemapFields f) {
   }
 }
 
+// moist_pt_last_step (moist_cv.py:139-162, dtmp = 0) on the last step,
+// adjust_divide (pt / pkz) otherwise, for the column at offset c
+__device__ __forceinline__ void final_pt(const RemapFields& f, int64_t c, int nz, int64_t sk) {
+  const fv3_remap_state& s = f.s;
+  double* pt = s.pt + c;
+  const double* pkz = s.pkz + c;
+  const double* qv = s.tracers[0] + c;
+  const double* ql = s.tracers[1] + c;
+  const double* qr = s.tracers[2] + c;
+  const double* qi = s.tracers[3] + c;
+  const double* qsn = s.tracers[4] + c;
+  const double* qg = s.tracers[5] + c;
+  if (f.p.last_step) {
+    const double dtmp = 0.0;
+    for (int k = 0; k <= nz; ++k) {
+      const int64_t o = k * sk;
+      const double gz = ql[o] + qr[o] + qi[o] + qsn[o] + qg[o];
+      pt[o] = (pt[o] + dtmp * pkz[o]) / ((1.0 + f.p.zvir * qv[o]) * (1.0 - gz));
+    }
+  } else {
+    for (int k = 0; k < nz; ++k) pt[k * sk] = pt[k * sk] / pkz[k * sk];
+  }
+}
+
 // post on [isc, iec] x [jsc, jec+1]: undo_delz_adjust_and_copy_peln
 // (remapping.py:43-53), moist_pkz (moist_cv.py:171-194), update_ua and
 // copy_from_below (remapping.py:182-198), then moist_pt_last_step
@@ -287,16 +311,16 @@ __global__ void __launch_bounds__(256) remap_post_kernel(Dom d, RemapFields f) {
   }
   for (int k = 0; k < nz; ++k) ua[k * sk] = pe2[(k + 1) * sk];
   for (int k = 1; k < nz; ++k) pe[k * sk] = pe2[k * sk];
-  if (f.p.last_step) {
-    const double dtmp = 0.0;
-    for (int k = 0; k <= nz; ++k) {
-      const int64_t o = k * sk;
-      const double gz = ql[o] + qr[o] + qi[o] + qsn[o] + qg[o];
-      pt[o] = (pt[o] + dtmp * pkz[o]) / ((1.0 + f.p.zvir * qv[o]) * (1.0 - gz));
-    }
-  } else {
-    for (int k = 0; k < nz; ++k) pt[k * sk] = pt[k * sk] / pkz[k * sk];
-  }
+  if (!f.p.defer_pt) final_pt(f, c, nz, sk);
+}
+
+// the final pt step alone (after a saturation adjustment) on [isc, iec] x [jsc, jec]
+__global__ void __launch_bounds__(256) remap_final_pt_kernel(Dom d, RemapFields f) {
+  const int ni = d.iec - d.isc + 1, nj = d.jec - d.jsc + 1;
+  const int n = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (n >= ni * nj) return;
+  const int i = d.isc + n % ni, j = d.jsc + n / ni;
+  final_pt(f, at3(d, i, j, 0, (int)blockIdx.y), d.nk, d.sk);
 }
 
 }  // namespace
@@ -402,6 +426,24 @@ int32_t fv3_lagrangian_to_eulerian_f64(const fv3_domain* dom, const fv3_remap_pa
   rc = map(p->kord_mt, -1, iec + 1, jec, s->v, work + kPe0v * work_slab,
            work + kPe3v * work_slab, nullptr, 0.0);
   if (rc) return rc;
+  return FV3_OK;
+}
+
+int32_t fv3_remap_final_pt_f64(const fv3_domain* dom, const fv3_remap_params* p,
+                               const fv3_remap_state* s, void* stream) {
+  using namespace fv3;
+  const char* who = "fv3_remap_final_pt";
+  int rc = check_domain(dom, who);
+  if (rc) return rc;
+  FV3_REQUIRE(p && s && s->pt && s->pkz, FV3_E_ARG, "%s: NULL pointer", who);
+  for (int n = 0; n < 6; ++n) FV3_REQUIRE(s->tracers[n], FV3_E_ARG, "%s: NULL tracer %d", who, n);
+  if (dom->ntiles == 0 || dom->iec < dom->isc || dom->jec < dom->jsc) return FV3_OK;
+  Dom d = to_dom(dom);
+  RemapFields f{*s, *p, nullptr, 0};
+  const int ncol = (d.iec - d.isc + 1) * (d.jec - d.jsc + 1);
+  hipLaunchKernelGGL(remap_final_pt_kernel, dim3((unsigned)((ncol + 255) / 256), d.ntiles),
+                     dim3(256), 0, as_stream(stream), d, f);
+  FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
 }
 

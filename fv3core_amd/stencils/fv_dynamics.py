@@ -130,9 +130,9 @@ class DynamicalCore:
             self._phis)
         self._hyperdiffusion = HyperdiffusionDamping(stencil_factory, damping_coefficients,
                                                      grid_data.rarea, config.nf_omega)
+        area_64 = grid_data.area_64 if grid_data.has("area_64") else grid_data.area
         self._lagrangian_to_eulerian_obj = LagrangianToEulerian(
-            stencil_factory, config.remapping, getattr(grid_data, "area_64", None), NQ,
-            self._pfull)
+            stencil_factory, config.remapping, area_64, NQ, self._pfull)
         self._omega_halo_updater = comm.get_scalar_halo_updater()
         if not self._skip_wrapup:
             self._cubed_to_latlon = CubedToLatLon(stencil_factory, grid_data,

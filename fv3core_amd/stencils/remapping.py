@@ -4,7 +4,8 @@
 kernel for the pressures and the moist pt conversion, the MapSingle remaps
 of pt, the tracers, w and delz, fillz, a column kernel for delz/peln/pkz,
 ua/pe and the final pt, then the remaps of u and v on their staggered
-pressures.  The component owns its scratch (pe1, pe2, dp2, pn2, the u/v
+pressures.  With do_sat_adj the saturation adjustment (SatAdjust3d) runs
+before the final pt step, which ``fv3_remap_final_pt`` then applies.  The component owns its scratch (pe1, pe2, dp2, pn2, the u/v
 pressures and the remap workspace), as the reference's does.
 """
 from __future__ import annotations
@@ -19,8 +20,10 @@ from .. import _lib
 from ..constants import CONSV_MIN
 from ..utils import gt4py_utils as utils
 from ..utils.stencil import StencilFactory
+from .._config import SatAdjustConfig
 from ._workspace import Workspace
 from .fillz import MAX_TRACERS
+from .saturation_adjustment import SatAdjust3d
 
 
 class RemapParams(ctypes.Structure):
@@ -29,7 +32,7 @@ class RemapParams(ctypes.Structure):
     _fields_ = [
         ("kord_tm", ctypes.c_int32), ("kord_tr", ctypes.c_int32), ("kord_wz", ctypes.c_int32),
         ("kord_mt", ctypes.c_int32), ("nq", ctypes.c_int32), ("fill", ctypes.c_int32),
-        ("last_step", ctypes.c_int32), ("pad_", ctypes.c_int32), ("ptop", ctypes.c_double),
+        ("last_step", ctypes.c_int32), ("defer_pt", ctypes.c_int32), ("ptop", ctypes.c_double),
         ("akap", ctypes.c_double), ("zvir", ctypes.c_double),
     ]
 
@@ -78,14 +81,14 @@ class LagrangianToEulerian:
         self._nq = int(nq)
         self._t_min = 184.0
         self._do_sat_adjust = bool(config.do_sat_adj)
-        if self._do_sat_adjust:
-            raise NotImplementedError("saturation adjustment (do_sat_adj) is not implemented")
         self.kmp = grid_indexing.domain[2] - 1
         pf = pfull.detach().cpu().numpy() if isinstance(pfull, torch.Tensor) else np.asarray(pfull)
         for k in range(pf.shape[0]):
             if pf[k] > 10.0e2:
                 self.kmp = k
                 break
+        self._saturation_adjustment = SatAdjust3d(
+            stencil_factory, getattr(config, "sat_adjust", SatAdjustConfig()), area_64, self.kmp)
         self._work = Workspace(15)
 
     def __call__(self, tracers: Dict[str, "torch.Tensor"], pt, delp, delz, peln, u, v, w, ua, va,
@@ -100,6 +103,11 @@ class LagrangianToEulerian:
             elif consv_te < -CONSV_MIN:
                 raise NotImplementedError(
                     f"Unimplemented/untested case consv({consv_te})  < -CONSV_MIN({-CONSV_MIN})")
+        fast_mp_consv = not do_adiabatic_init and consv_te > CONSV_MIN
+        if self._do_sat_adjust and fast_mp_consv:
+            # the column energy sum after the adjustment (sum_te, remapping.py:199-202,
+            # 607-611) is not built
+            raise NotImplementedError("consv_te > CONSV_MIN with do_sat_adj is not implemented")
         nz = self._gi.domain[2]
         names = utils.tracer_variables
         fields3 = [pt, delp, delz, peln, u, v, w, ua, cappa, q_con, pkz, pk, pe]
@@ -112,7 +120,8 @@ class LagrangianToEulerian:
         p = RemapParams(kord_tm=int(self._config.kord_tm), kord_tr=int(self._config.kord_tr),
                         kord_wz=int(self._config.kord_wz), kord_mt=int(self._config.kord_mt),
                         nq=self._nq, fill=int(bool(self._config.fill)),
-                        last_step=int(bool(last_step)), pad_=0, ptop=float(ptop),
+                        last_step=int(bool(last_step)), defer_pt=int(self._do_sat_adjust),
+                        ptop=float(ptop),
                         akap=float(akap), zvir=float(zvir))
         akd = _column(ak, dev, nz + 1)
         bkd = _column(bk, dev, nz + 1)
@@ -125,3 +134,13 @@ class LagrangianToEulerian:
         _lib.check(_lib.lib().fv3_lagrangian_to_eulerian_f64(
             ctypes.byref(d), ctypes.byref(p), ctypes.byref(s), _lib.ptr(work),
             ctypes.c_int64(slab), ctypes.c_void_p(_lib.stream_handle())))
+        if self._do_sat_adjust:
+            # remapping.py:581-606: the adjustment between the remap and the
+            # final pt step, with dp1 as its energy scratch
+            self._saturation_adjustment(
+                dp1, tracers["qvapor"], tracers["qliquid"], tracers["qice"], tracers["qrain"],
+                tracers["qsnow"], tracers["qgraupel"], q_cld, hs, peln, delp, delz, q_con, pt,
+                pkz, cappa, zvir, mdt, fast_mp_consv, last_step, akap, self.kmp)
+            _lib.check(_lib.lib().fv3_remap_final_pt_f64(
+                ctypes.byref(d), ctypes.byref(p), ctypes.byref(s),
+                ctypes.c_void_p(_lib.stream_handle())))

@@ -581,6 +581,44 @@ int32_t fv3_dyncore_column_f64(const fv3_domain* d, int32_t op, double scalar, c
 int32_t fv3_dyncore_column_f32(const fv3_domain* d, int32_t op, double scalar, const float* a,
                                const float* b, float* out, void* stream);
 
+/* ---- fast saturation adjustment ----------------------------------------- */
+
+/* Namelist values and per-call scalars of SatAdjust3d
+ * (fv3core/stencils/saturation_adjustment.py:899-1024). */
+typedef struct fv3_satadj_params {
+  int32_t hydrostatic, rad_snow, rad_rain, rad_graupel, tintqs, icloud_f;
+  int32_t do_qa, consv_te, last_step;
+  double sat_adj0, ql_gen, qs_mlt, ql0_max, t_sub, qi_gen, qi_lim, qi0_max;
+  double dw_ocean, dw_land, cld_min;
+  double sdt, zvir, fac_i2s, c_air, c_vap, mdt, fac_r2g, fac_smlt, fac_l2r, fac_imlt;
+  double d0_vap, lv00, fac_v2l, fac_l2v;
+} fv3_satadj_params;
+
+/*
+ * The saturation vapour-pressure tables of saturation_adjustment.py:43-168
+ * (qs_tablew_fn, qs_table2_fn, desw_table, des2_table) for the table indices
+ * -1 .. 2620: four consecutive arrays of fv3_sat_adjust_table_entries()
+ * doubles, entry e holding index e - 1.  Each value is the closed form the
+ * stencil evaluates at that index.
+ */
+int32_t fv3_sat_adjust_table_entries(void);
+int32_t fv3_sat_adjust_tables_f64(double* tables, void* stream);
+
+/*
+ * SatAdjust3d.__call__ (saturation_adjustment.py:936-1024): the satadjust
+ * stencil (:554-896) on [isc, iec] x [jsc, jec] x [kmp, d->nk), FP64.
+ * In/out: qv, ql, qi, qr, qs, qg (qvapor .. qgraupel), qa (qcld, written on
+ * the last step), pt, q_con, cappa, pkz (nonhydrostatic), te (written when
+ * consv_te), delz (shifted down one level in the hydrostatic form, as the
+ * stencil does).  area, hs: IJ; peln: d->nk + 1 levels (hydrostatic only).
+ */
+int32_t fv3_sat_adjust_f64(const fv3_domain* d, int32_t kmp, const fv3_satadj_params* p,
+                           const double* tables, double* te, double* qv, double* ql, double* qi,
+                           double* qr, double* qs, double* qg, double* qa, const double* area,
+                           const double* hs, const double* peln, const double* delp,
+                           double* delz, double* q_con, double* pt, double* pkz, double* cappa,
+                           void* stream);
+
 /* ---- vertical remapping (Lagrangian -> Eulerian) ---------------------- */
 
 /* Most tracers a remapping call handles (fv3core/utils/gt4py_utils.py:26-36). */
@@ -603,7 +641,8 @@ typedef struct fv3_remap_params {
   int32_t nq;                                 /* tracers remapped (<= FV3_MAX_TRACERS) */
   int32_t fill;                               /* fillz after the tracer remap */
   int32_t last_step;                          /* moist_pt_last_step instead of pt / pkz */
-  int32_t pad_;
+  int32_t defer_pt;  /* nonzero: leave the final pt step (moist_pt_last_step or pt / pkz) to
+                        fv3_remap_final_pt_f64, so that a saturation adjustment runs first */
   double ptop, akap, zvir;
 } fv3_remap_params;
 
@@ -621,7 +660,9 @@ typedef struct fv3_remap_state {
 
 /*
  * LagrangianToEulerian.__call__ (fv3core/stencils/remapping.py:416-628)
- * for kord_tm < 0, non-hydrostatic, nwat = 6, no saturation adjustment:
+ * for kord_tm < 0, non-hydrostatic, nwat = 6 (the saturation adjustment,
+ * when on, runs between this call with p->defer_pt set and
+ * fv3_remap_final_pt_f64):
  * remaps pt (log-pressure, T >= 184 K), the nq tracers (then fillz), w,
  * delz, u and v from the Lagrangian interfaces pe to the hybrid levels
  * ak + bk * ps, updates delp, peln, pk, pkz, cappa, q_con, ps, pe, ua.
@@ -631,6 +672,12 @@ typedef struct fv3_remap_state {
 int32_t fv3_lagrangian_to_eulerian_f64(const fv3_domain* d, const fv3_remap_params* p,
                                        const fv3_remap_state* s, double* work,
                                        int64_t work_slab, void* stream);
+
+/* The final pt step of LagrangianToEulerian.__call__ (remapping.py:613-628):
+ * moist_pt_last_step (moist_cv.py:139-162, dtmp = 0) on the last step, else
+ * pt / pkz, on the compute domain; for calls made with p->defer_pt. */
+int32_t fv3_remap_final_pt_f64(const fv3_domain* d, const fv3_remap_params* p,
+                               const fv3_remap_state* s, void* stream);
 
 /* ---- full dynamics step (fv_dynamics) --------------------------------- */
 

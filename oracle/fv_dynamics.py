@@ -60,7 +60,8 @@ def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, 
     dp1, cvm, wsd, te0_2d, ut, vt, gz, zh, pem, pkc, pk3, heat_source, divgd,
     crx, xfx, cry, yfx, ws3, delpc, ptc).  ``cfg``: namelist values plus bdt
     (the step), ptop, ks, k_split, n_split, hord_tr, kord_*, fill, z_tracer,
-    nf_omega, p_ref.  ``halo(kind, names)`` refreshes state halos in place;
+    nf_omega, p_ref, and optionally do_sat_adj with sat_adjust (a
+    SatAdjustConfig).  ``halo(kind, names)`` refreshes state halos in place;
     ``tracer_halo(list)`` returns refreshed copies; ``sync()`` joins the
     D-grid edge winds.  ``wrapup``: end with neg_adj3 and CubedToLatLon
     (order ``cfg["c2l_ord"]``, default 4; the metrics hold a11..a22).
@@ -93,11 +94,21 @@ def fv_dynamics(S, cfg, metrics, scalars, ak, bk, halo, tracer_halo, sync, col, 
                         kord_mt=cfg["kord_mt"], fill=cfg["fill"])
             names = ("pt", "delp", "delz", "peln", "u", "v", "w", "ua", "cappa", "q_con", "pkz",
                      "pk", "pe", "ps", "wsd")
+            sat_cfg = cfg.get("sat_adjust") if cfg.get("do_sat_adj") else None
+            kmp = nz - 1
+            for k in range(nz):
+                if pfull[k] > 10.0e2:
+                    kmp = k
+                    break
             for ti in range(ntiles):
                 st = {k: S[k][ti] for k in names}
                 trs = {k: S[k][ti] for k in orm.TRACERS}
+                sa = None
+                if sat_cfg is not None:
+                    sa = dict(config=sat_cfg, area=metrics[ti]["area"], hs=S["phis"][ti],
+                              dp1=S["dp1"][ti], mdt=cfg["bdt"] / k_split, kmp=kmp)
                 o, q = orm.lagrangian_to_eulerian(st, trs, ak, bk, cfg["ptop"], KAPPA, ZVIR,
-                                                  last_step, rcfg, NQ, t)
+                                                  last_step, rcfg, NQ, t, sat_adjust=sa)
                 for k in names:
                     S[k][ti][...] = o[k]
                 for k in orm.TRACERS:

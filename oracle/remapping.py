@@ -134,14 +134,17 @@ def _map(q, pe1, pe2, kord, mode, qmin=0.0, qs=None):
 
 
 def lagrangian_to_eulerian(st, tracers, ak, bk, ptop, akap, zvir, last_step, cfg, nq, t,
-                           consv_te=0.0, do_adiabatic_init=False):
+                           consv_te=0.0, do_adiabatic_init=False, sat_adjust=None):
     """LagrangianToEulerian.__call__ (remapping.py:416-628) on one tile.
 
     ``st``: dict of tile arrays [I, J, nz + 1] (pt, delp, delz, peln, u, v, w,
     ua, cappa, q_con, pkz, pk, pe) and IJ arrays (ps, wsd); ``tracers``: dict
     of [I, J, nz + 1] arrays by name.  ``cfg``: kord_tm, kord_tr, kord_wz,
     kord_mt, fill.  ``t``: compute-domain bounds (isc, iec, jsc, jec).
-    Returns updated copies (st, tracers).
+    ``sat_adjust``: None, or the saturation adjustment's inputs (config,
+    area, hs, dp1 — its energy scratch, updated in place — mdt, kmp), run
+    before the final pt step as remapping.py:581-606 does (tracers must then
+    hold qcld).  Returns updated copies (st, tracers).
     """
     st = {k: np.array(v, dtype=np.float64, copy=True) for k, v in st.items()}
     tr = {k: np.array(v, dtype=np.float64, copy=True) for k, v in tracers.items()}
@@ -238,6 +241,21 @@ def lagrangian_to_eulerian(st, tracers, ak, bk, ptop, akap, zvir, last_step, cfg
     pe[C + (slice(1, nz),)] = ua[C][..., 0:nz - 1]
     if last_step and not do_adiabatic_init and abs(consv_te) > CONSV_MIN:
         raise NotImplementedError("consv_te beyond CONSV_MIN is not supported")
+    if sat_adjust is not None:
+        from .satadjust import sat_adjust_3d
+
+        sa = sat_adjust
+        out = sat_adjust_3d(sa["config"], sa["area"], sa["dp1"], tr["qvapor"], tr["qliquid"],
+                            tr["qice"], tr["qrain"], tr["qsnow"], tr["qgraupel"], tr["qcld"],
+                            sa["hs"], st["peln"], st["delp"], st["delz"], st["q_con"], st["pt"],
+                            st["pkz"], st["cappa"], zvir, sa["mdt"], False, last_step,
+                            sa["kmp"], (i0, i1, j0, j1), nz)
+        for name, key in (("qvapor", "qv"), ("qliquid", "ql"), ("qice", "qi"), ("qrain", "qr"),
+                          ("qsnow", "qs"), ("qgraupel", "qg"), ("qcld", "qa")):
+            tr[name] = out[key]
+        for name in ("pt", "q_con", "cappa", "pkz", "delz"):
+            st[name] = out[name]
+        sa["dp1"][...] = out["te0"]
     if last_step:
         # moist_pt_last_step (moist_cv.py:143-169) on levels 0..nz, dtmp = 0
         g = (tr["qliquid"][C][..., Ki] + tr["qrain"][C][..., Ki] + tr["qice"][C][..., Ki]

@@ -21,11 +21,19 @@ N, NZ = 12, 8
 TRACERS = orm.TRACERS
 
 
-def moist_cube_state(n, nz, seed):
+def moist_cube_state(n, nz, seed, temperature=False):
     """cube_state plus the nine tracers: vapour decreasing upwards, small
-    condensates, and a qliquid that changes sign so that fillz acts."""
+    condensates, and a qliquid that changes sign so that fillz acts.  With
+    ``temperature``, pt holds the temperature, as step_dynamics' callers pass
+    it (its preamble divides by pkz), instead of cube_state's T / p**kappa,
+    so that the saturation adjustment sees 216-288 K."""
     init = cube_state(n, nz, seed=seed)
     st = init["state"]
+    if temperature:
+        pe = st["pe"]
+        pm = st["delp"][..., :nz] / (np.log(pe[..., 1:nz + 1]) - np.log(pe[..., :nz]))
+        st["pt"] = st["pt"].copy()
+        st["pt"][..., :nz] = st["pt"][..., :nz] * np.exp(KAPPA * np.log(pm))
     rng = np.random.default_rng(seed + 100)
     shp = st["delp"].shape
     lev = np.linspace(0.2, 1.0, shp[-1])[None, None, None, :]
@@ -168,7 +176,8 @@ def _oracle_step(init, cfg, bdt, ptop, seed, wrapup=False):
                 p_fac=cfg.p_fac, rf_cutoff=cfg.rf_cutoff, tau=cfg.tau, delt_max=cfg.delt_max,
                 kord_tm=cfg.kord_tm, kord_tr=cfg.kord_tr, kord_wz=cfg.kord_wz,
                 kord_mt=cfg.kord_mt, fill=cfg.fill, z_tracer=cfg.z_tracer,
-                nf_omega=cfg.nf_omega, p_ref=cfg.p_ref, c2l_ord=cfg.c2l_ord)
+                nf_omega=cfg.nf_omega, p_ref=cfg.p_ref, c2l_ord=cfg.c2l_ord,
+                do_sat_adj=cfg.do_sat_adj, sat_adjust=cfg.sat_adjust)
     metrics = [{k: v[ti] for k, v in m.items() if isinstance(v, np.ndarray) and v.ndim == 3}
                for ti in range(6)]
     ofd.fv_dynamics(S, ocfg, metrics, dict(da_min=m["da_min"], da_min_c=m["da_min_c"]), ak,
@@ -177,9 +186,11 @@ def _oracle_step(init, cfg, bdt, ptop, seed, wrapup=False):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("k_split,kord,wrapup", [(1, 9, False), (2, 9, False), (1, 10, False),
-                                                 (2, 10, False), (1, 9, True), (2, 9, True)])
-def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup):
+@pytest.mark.parametrize("k_split,kord,wrapup,sat_adj",
+                         [(1, 9, False, False), (2, 9, False, False), (1, 10, False, False),
+                          (2, 10, False, False), (1, 9, True, False), (2, 9, True, False),
+                          (1, 9, False, True), (2, 9, True, True)])
+def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup, sat_adj):
     """kord 9: every field within 1e-9 (w and omga 1e-7).  kord 10: its
     extremum tests (|x0| > |a2 - a3|, remap_profile.py:271-277) switch the
     reconstruction discontinuously, so an input that sits within last-bit
@@ -191,19 +202,21 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup):
     oracle and the oracle on inputs moved by one ulp (plus 1%), and the column
     integrals of delz and of every tracer's mass must agree to 1e-9.  With
     the wrap-up (neg_adj3 and CubedToLatLon), qcld, ua and va are compared
-    too."""
+    too; with the saturation adjustment (do_sat_adj, inside every remap),
+    qcld as well."""
     from fv3core_amd.halo import CubedSphereCommunicator
     from fv3core_amd.stencils.fv_dynamics import DynamicalCore
     from fv3core_amd.utils import gt4py_utils as utils
     from fv3core_amd.utils.grid import DampingCoefficients, GridData
     from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
 
-    init = moist_cube_state(N, NZ, seed=4)
+    init = moist_cube_state(N, NZ, seed=4, temperature=sat_adj)
     st0 = init["state"]
     tr0 = init["tracers"]
     m = init["metric"]
     ak, bk, ptop = init["ak"], init["bk"], init["ptop"]
     cfg = _config(k_split=k_split, kord=kord)
+    cfg.do_sat_adj = sat_adj
     gi = GridIndexing((N, N, NZ), 3, True, True, True, True)
     factory = StencilFactory(StencilConfig(), gi)
 
@@ -262,8 +275,9 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup):
     check("ps", (slice(None), slice(3, N + 3), slice(3, N + 3)), 1e-12)
     for name in ("mfxd", "mfyd", "cxd", "cyd"):
         check(name, c)
-    if wrapup:
+    if wrapup or sat_adj:
         check("qcld", c)
+    if wrapup:
         # ua and va are sums of u and v terms with O(1) weights: where the
         # terms cancel, their absolute error is that of u and v (1e-9 of the
         # wind scale), so points below 1e-4 of the field's scale are held to
