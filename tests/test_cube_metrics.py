@@ -28,13 +28,149 @@ def test_areas_cover_the_sphere(n):
     np.testing.assert_allclose(total, 4.0 * math.pi * cube.RADIUS ** 2, rtol=1e-12)
 
 
+def _not_corner_blocks(n):
+    """Cells outside the three-tile corner blocks of the halo (where the
+    reference leaves its 1e8 / 1e-8 placeholders)."""
+    nc = n + 2 * H
+    ok = np.ones((nc, nc), dtype=bool)
+    for a in (slice(0, H), slice(nc - H, nc)):
+        for b in (slice(0, H), slice(nc - H, nc)):
+            ok[a, b] = False
+    return ok
+
+
 def test_corner_angles_are_angles():
-    """sin_sg^2 + cos_sg^2 = 1 for the four cell-corner angle terms and
-    sina^2 + cosa^2 = 1 at the corners, on the whole storage."""
-    m = cube.tile_metrics(12)
-    for k in (1, 2, 3, 4):
-        np.testing.assert_allclose(m[f"sin_sg{k}"] ** 2 + m[f"cos_sg{k}"] ** 2, 1.0, rtol=1e-13)
-    np.testing.assert_allclose(m["sina"] ** 2 + m["cosa"] ** 2, 1.0, rtol=1e-13)
+    """sin_sg^2 + cos_sg^2 = 1 for the nine supergrid angle terms on every
+    cell outside the corner blocks, and sina^2 + cosa^2 = 1 at the
+    corners inside the tile (on the tile edges they average a cell of each
+    tile, whose angles differ by the kink)."""
+    n = 12
+    m = cube.tile_metrics(n)
+    ok = _not_corner_blocks(n)
+    nc = n + 2 * H
+    for k in range(1, 10):
+        s = m[f"sin_sg{k}"][:, :nc, :nc][:, ok]
+        c = m[f"cos_sg{k}"][:, :nc, :nc][:, ok]
+        np.testing.assert_allclose(s ** 2 + c ** 2, 1.0, rtol=1e-13, err_msg=f"sg{k}")
+    w = (slice(None), slice(H + 1, H + n), slice(H + 1, H + n))
+    # cosa/sina average the two cells' corner angles, so the identity holds
+    # to the square of their difference
+    np.testing.assert_allclose(m["sina"][w] ** 2 + m["cosa"][w] ** 2, 1.0, atol=3e-3)
+
+
+def _nearest(points, targets):
+    """Index and distance of the nearest target (rows) to each point."""
+    d = np.linalg.norm(points[:, None, :] - targets[None, :, :], axis=-1)
+    k = np.argmin(d, axis=1)
+    return k, d[np.arange(len(points)), k]
+
+
+def test_halo_points_are_the_neighbouring_tiles_points():
+    """Every edge-halo grid point of a tile is a compute-domain grid point of
+    another tile (the reference's halo update of the grid,
+    generation.py:1355), not the continuation of the tile's own face: the
+    own-face extension misses the neighbours' points by a visible fraction
+    of a cell."""
+    n = 12
+    P = cube.grid_points(n, H)
+    own = cube._turn(cube.tile_corners(n, H), cube.LON_SHIFT)
+    comp = P[:, H:H + n + 1, H:H + n + 1].reshape(-1, 3)
+    e = H + n
+    for t in range(6):
+        for name, blk in (("west", (slice(0, H), slice(H, e + 1))),
+                          ("east", (slice(e + 1, e + 1 + H), slice(H, e + 1))),
+                          ("south", (slice(H, e + 1), slice(0, H))),
+                          ("north", (slice(H, e + 1), slice(e + 1, e + 1 + H)))):
+            pts = P[t][blk].reshape(-1, 3)
+            k, d = _nearest(pts, comp)
+            assert d.max() < 1e-14, (t, name, d.max())
+            assert np.all(k // (n + 1) ** 2 != t), (t, name)  # another tile's points
+            _, d_own = _nearest(own[t][blk].reshape(-1, 3), comp)
+            assert d_own.max() > 1e-3 * (2 * math.pi / (4 * n)), (t, name)
+
+
+def test_three_tile_corner_blocks_of_the_grid():
+    """At a cube corner three tiles meet: the south halo's first point beside
+    the corner and the west halo's first point beside it are the same point
+    (the first point along the third edge), and the corner cell's angle at
+    the cube corner is a third of a full turn (cos = -1/2, the cube's
+    spherical corner of 120 degrees)."""
+    n = 12
+    m = cube.tile_metrics(n)
+    P = m["grid3"]
+    e = H + n
+    np.testing.assert_allclose(P[:, H, H - 1], P[:, H - 1, H], atol=1e-15)
+    np.testing.assert_allclose(P[:, e, e + 1], P[:, e + 1, e], atol=1e-15)
+    np.testing.assert_allclose(m["cos_sg6"][:, H, H], -0.5, atol=1e-13)
+    np.testing.assert_allclose(m["cos_sg8"][:, e - 1, e - 1], -0.5, atol=1e-13)
+
+
+def test_halo_lengths_and_areas_are_the_neighbours():
+    """dx, dy, area and the damping coefficients in the edge halos equal the
+    neighbouring tile's compute-domain values at the same place: the halo
+    dx / dy are the great-circle lengths between the halo grid points, the
+    halo areas the spherical areas of the halo cells, and area_c on the tile
+    edges is the same on both tiles sharing the edge."""
+    n = 12
+    m = cube.tile_metrics(n)
+    P = m["grid3"]
+    e = H + n
+    R = cube.RADIUS
+    west = (slice(None), slice(0, H), slice(H, e))
+    dy_geo = R * cube._gc_dist(P[:, :, :-1], P[:, :, 1:])
+    np.testing.assert_allclose(m["dy"][:, 0:H, H:e], dy_geo[:, 0:H, H:e], rtol=1e-12)
+    dx_geo = R * cube._gc_dist(P[:, :-1, :], P[:, 1:, :])
+    np.testing.assert_allclose(m["dx"][west], dx_geo[west], rtol=1e-12)
+    area_geo = R ** 2 * cube._sph_area(P[:, :-1, :-1], P[:, 1:, :-1], P[:, 1:, 1:], P[:, :-1, 1:])
+    np.testing.assert_allclose(m["area"][west], area_geo[west], rtol=1e-10)
+    # tile 0's east edge is tile 1's west edge, aligned
+    np.testing.assert_allclose(m["area_c"][0, e, H:e + 1], m["area_c"][1, H, H:e + 1], rtol=1e-12)
+    np.testing.assert_allclose(m["dxc"][0, e, H:e], m["dxc"][1, H, H:e], rtol=1e-12)
+    # the halo dxc next to tile 0's east edge is tile 1's first interior dxc
+    np.testing.assert_allclose(m["dxc"][0, e + 1, H:e], m["dxc"][1, H + 1, H:e], rtol=1e-12)
+    np.testing.assert_allclose(m["del6_v"][0, e + 1, H:e], m["del6_v"][1, H + 1, H:e], rtol=1e-12)
+
+
+def test_tile_edge_c_grid_lengths():
+    """On a tile edge the C-grid length is twice the distance from the edge
+    midpoint to the tile's own cell centre (gnomonic.py:546-590).  The two
+    cells across the edge are mirror images in the edge's plane, so the
+    distance between their centres is twice the distance from the centre to
+    its foot on the edge, which is no farther than the midpoint: the edge
+    length is at least the centre-to-centre length, equal to it only where
+    the foot is the midpoint (near the middle of the edge)."""
+    n = 12
+    m = cube.tile_metrics(n)
+    P, C = m["grid3"], m["agrid3"]
+    e = H + n
+    mid = cube._normalize(P[:, H, H:e] + P[:, H, H + 1:e + 1])
+    np.testing.assert_allclose(m["dxc"][:, H, H:e],
+                               2 * cube.RADIUS * cube._gc_dist(mid, C[:, H, H:e]), rtol=1e-13)
+    chord = cube.RADIUS * cube._gc_dist(C[:, H - 1, H:e], C[:, H, H:e])
+    r = m["dxc"][:, H, H:e] / chord
+    assert np.all(r >= 1.0 - 1e-12) and np.all(r < 1.2)
+    assert r[:, n // 2].max() < 1.01 and r[:, 0].min() > 1.05
+    np.testing.assert_allclose(r, r[:, ::-1], rtol=1e-12)  # symmetric along the edge
+
+
+def test_angle_terms_agree_across_an_edge_midpoint():
+    """The east-edge angle of a cell and the west-edge angle of the next cell
+    sit at the same edge midpoint between the same two grid lines, so they
+    agree closely (exactly on a grid of great circles through the centres);
+    cosa_u, their average, is therefore close to both.  A sign slip in
+    either term turns cosa_u to about zero."""
+    n = 24
+    m = cube.tile_metrics(n)
+    c = (slice(None), slice(H, H + n - 1), slice(H, H + n))
+    c_next = (slice(None), slice(H + 1, H + n), slice(H, H + n))
+    np.testing.assert_allclose(m["cos_sg3"][c], m["cos_sg1"][c_next], atol=2e-3)
+    np.testing.assert_allclose(m["cosa_u"][c_next], m["cos_sg1"][c_next], atol=1e-3)
+    c = (slice(None), slice(H, H + n), slice(H, H + n - 1))
+    c_next = (slice(None), slice(H, H + n), slice(H + 1, H + n))
+    np.testing.assert_allclose(m["cos_sg4"][c], m["cos_sg2"][c_next], atol=2e-3)
+    np.testing.assert_allclose(m["cosa_v"][c_next], m["cos_sg2"][c_next], atol=1e-3)
+    # the corner angle of a cell is close to its edge angles' mean away from the edges
+    assert np.abs(m["cosa_u"][c_next]).max() > 0.2
 
 
 def test_six_tiles_are_rotations_of_one():

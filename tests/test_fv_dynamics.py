@@ -111,8 +111,8 @@ def test_oracle_kord10_step_switches_keep_column_integrals():
     """The reason the kord-10 step cases are judged by column integrals and
     a share of points: moving pt, delp, u, v and the tracers by one ulp (random
     signs) before a k_split = 2 step flips kord-10 extremum tests in the
-    oracle itself, which moves delz at several percent of the points by up to
-    ~1e-4 (relative), while every column integral of delz and of each
+    oracle itself, which moves delz at several percent of the points by
+    1e-6 to 1e-4 (relative, depending on the state), while every column integral of delz and of each
     tracer's mass moves by less than 1e-8 and pt, delp by less than 1e-9
     (the second remap step acts on the first one's output, so the integrals
     are not exact invariants at k_split = 2: the GPU test allows them 10x
@@ -128,7 +128,7 @@ def test_oracle_kord10_step_switches_keep_column_integrals():
         return 2.0 * np.abs(a - b) / np.maximum(np.abs(a) + np.abs(b), 2e-6 * np.abs(a).max())
 
     e = rel(S["delz"][c], P["delz"][c])
-    assert e.max() > 1e-5 and (e > 1e-9).mean() > 0.01
+    assert e.max() > 1e-6 and (e > 1e-9).mean() > 0.01
     for name in ("pt", "delp"):
         assert rel(S[name][c], P[name][c]).max() < 1e-9
     assert rel(S["delz"][c].sum(-1), P["delz"][c].sum(-1)).max() < 1e-8
