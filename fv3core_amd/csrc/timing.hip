@@ -19,6 +19,7 @@ struct Pair {
 std::mutex g_mu;
 std::string g_armed;
 bool g_any_armed = false;
+bool g_held = false;
 std::vector<Pair> g_pool;
 size_t g_used = 0;
 
@@ -56,8 +57,17 @@ int32_t fv3_timing_arm(const char* kernel_name) {
   using namespace fv3;
   std::lock_guard<std::mutex> lk(g_mu);
   g_armed = kernel_name ? kernel_name : "";
+  g_held = false;
   g_any_armed = !g_armed.empty();
   g_used = 0;
+  return FV3_OK;
+}
+
+int32_t fv3_timing_hold(int32_t hold) {
+  using namespace fv3;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_held = hold != 0;
+  g_any_armed = !g_armed.empty() && !g_held;
   return FV3_OK;
 }
 

@@ -63,9 +63,13 @@ int32_t fv3_device_count(void);
  * it); fv3_timing_read waits for those events and returns the number of
  * timed launches, their summed duration and their summed algorithmic HBM
  * bytes (inputs read once, outputs written once), then starts a new count.
+ * fv3_timing_hold(1) suspends the recording without clearing the count and
+ * fv3_timing_hold(0) resumes it, so that a benchmark can time the launches
+ * of one part of its timed region.
  * Timed kernels: fvtp2d_kernel, map_single_kernel, riem_c_kernel,
  * riem3_kernel. */
 int32_t fv3_timing_arm(const char* kernel_name);
+int32_t fv3_timing_hold(int32_t hold);
 int32_t fv3_timing_read(int64_t* count, double* total_ms, double* total_bytes);
 
 /*
