@@ -741,7 +741,7 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": _profiled_traffic(roofline_kernel, "ensemble"),
             "kernel": roofline_kernel,
             "kernel_launches_timed": cnt,
             "kernel_launches_of": "the 32-member C48 batch (mixed form, eager timed steps)",
@@ -876,6 +876,8 @@ def main():
         cpu = None if args.no_cpu_baseline else cpu_baseline_dyncore(args.cpu_seconds)
         res = bench_ensemble(steps, warmup, not args.no_graph, args.roofline_kernel,
                              adjoint=not args.no_adjoint)
+        if args.copy_calibration:
+            res["copy_calibration_bytes"] = _copy_calibration()
     elif args.workload == "adjoint":
         steps = 10 if args.steps is None else args.steps
         warmup = 2 if args.warmup is None else args.warmup

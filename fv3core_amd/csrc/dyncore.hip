@@ -79,11 +79,13 @@ __device__ __forceinline__ void floor_column_up(T* __restrict__ h, int64_t sk, i
 // a per-column offset, formed once.
 constexpr int KZ = 4;
 
-template <typename T>
+// U is the storage type of ut and vt: FP64, or FP32 in the mixed-precision
+// step (widened exactly on load, so the sums are the FP64 kernel's).
+template <typename T, typename U>
 __global__ void __launch_bounds__(256)
     update_dz_c_kernel(Dom d, const T* __restrict__ dp0, const T* __restrict__ zs,
-                       const T* __restrict__ area, const T* __restrict__ ut,
-                       const T* __restrict__ vt, const T* __restrict__ gz, T* __restrict__ gzn,
+                       const T* __restrict__ area, const U* __restrict__ ut,
+                       const U* __restrict__ vt, const T* __restrict__ gz, T* __restrict__ gzn,
                        T* __restrict__ ws, T rdt) {
   const int ni = d.iec - d.isc + 3, nj = d.jec - d.jsc + 3;
   const int n = (int)(blockIdx.x * 256 + threadIdx.x);
@@ -121,10 +123,10 @@ __global__ void __launch_bounds__(256)
     for (int u = 0; u < KZ; ++u) {
       const int64_t ol = oc + (int64_t)min(k0 + u, nk - 2) * sk;  // layer
       const int64_t og = oc + (int64_t)min(k0 + u, nk - 1) * sk;  // interface
-      u0[u] = ut[ol];
-      u1[u] = ut[ol + 1];
-      v0[u] = vt[ol];
-      v1[u] = vt[ol + sj];
+      u0[u] = T(ut[ol]);
+      u1[u] = T(ut[ol + 1]);
+      v0[u] = T(vt[ol]);
+      v1[u] = T(vt[ol + sj]);
       gx[u][0] = gz[og + gxm];
       gx[u][1] = gz[og + gx0];
       gx[u][2] = gz[og + gxp];
@@ -165,32 +167,34 @@ __global__ void __launch_bounds__(256)
 // ----------------------------------------------------------------- updatedzd
 // cubic_spline_interpolation_from_layer_center_to_interfaces for up to four
 // fields, one column per lane over the full domain.
-template <typename T>
+// U is the storage type of the layer values (FP64, or FP32 in the
+// mixed-precision step, widened exactly on load).
+template <typename T, typename U>
 __global__ void __launch_bounds__(256)
     spline_kernel(Dom d, const T* __restrict__ gk, const T* __restrict__ beta,
-                  const T* __restrict__ gamma, const T* q0, const T* q1, const T* q2,
-                  const T* q3, T* o0, T* o1, T* o2_, T* o3) {
+                  const T* __restrict__ gamma, const U* q0, const U* q1, const U* q2,
+                  const U* q3, T* o0, T* o1, T* o2_, T* o3) {
   const int ni = d.iec - d.isc + 7, nj = d.jec - d.jsc + 7;
   const int n = (int)(blockIdx.x * 256 + threadIdx.x);
   if (n >= ni * nj) return;
   const int i = d.isc - 3 + n % ni, j = d.jsc - 3 + n / ni, t = (int)blockIdx.y;
   const int f = (int)blockIdx.z;
-  const T* qc = f == 0 ? q0 : f == 1 ? q1 : f == 2 ? q2 : q3;
+  const U* qc = f == 0 ? q0 : f == 1 ? q1 : f == 2 ? q2 : q3;
   T* qi = f == 0 ? o0 : f == 1 ? o1 : f == 2 ? o2_ : o3;
   const int nk = d.nk;
   const int64_t base = at3(d, i, j, 0, t), sk = d.sk;
   T prev;
   {
     const T xt1 = T(2) * gk[0] * (gk[0] + T(1));
-    prev = (xt1 * qc[base] + qc[base + sk]) / beta[0];
+    prev = (xt1 * T(qc[base]) + T(qc[base + sk])) / beta[0];
     qi[base] = prev;
   }
-  T q_prev = qc[base];  // qc[k - 1]
+  T q_prev = T(qc[base]);  // qc[k - 1]
   for (int k0 = 1; k0 < nk - 1; k0 += KC) {
     T qv[KC];
 #pragma unroll
     for (int u = 0; u < KC; ++u)
-      if (k0 + u < nk - 1) qv[u] = qc[base + (k0 + u) * sk];
+      if (k0 + u < nk - 1) qv[u] = T(qc[base + (k0 + u) * sk]);
 #pragma unroll
     for (int u = 0; u < KC; ++u) {
       const int k = k0 + u;
@@ -207,7 +211,7 @@ __global__ void __launch_bounds__(256)
     const T a_bot = T(1) + g * (g + T(1.5));
     const T xt1 = T(2) * g * (g + T(1));
     const T xt2 = g * (g + T(0.5)) - a_bot * gamma[k - 1];
-    prev = (xt1 * qc[base + (k - 1) * sk] + qc[base + (k - 2) * sk] - a_bot * prev) / xt2;
+    prev = (xt1 * T(qc[base + (k - 1) * sk]) + T(qc[base + (k - 2) * sk]) - a_bot * prev) / xt2;
     qi[base + k * sk] = prev;
   }
   for (int k0 = nk - 2; k0 >= 0; k0 -= KC) {
@@ -685,15 +689,12 @@ int check_metrics_list(const fv3_metrics* g, const void* const* ptrs, int n, con
   return FV3_OK;
 }
 
-}  // namespace fv3
-
-extern "C" {
-
-int32_t fv3_update_dz_c_f64(const fv3_domain* dom, const double* area, const double* zs,
-                            const double* dp_ref, const double* ut, const double* vt,
-                            double* gz, double* ws, double dt, double* work, int64_t work_slab,
-                            void* stream) {
-  using namespace fv3;
+// UpdateGeopotentialHeightOnCGrid (updatedzc.py) with the layer winds ut, vt
+// of storage type U.
+template <typename U>
+int update_dz_c(const fv3_domain* dom, const double* area, const double* zs,
+                const double* dp_ref, const U* ut, const U* vt, double* gz, double* ws,
+                double dt, double* work, int64_t work_slab, void* stream) {
   const char* who = "fv3_update_dz_c";
   int rc = check_domain(dom, who);
   if (rc) return rc;
@@ -706,8 +707,8 @@ int32_t fv3_update_dz_c_f64(const fv3_domain* dom, const double* area, const dou
   Dom d = to_dom(dom);
   hipStream_t s = as_stream(stream);
   const int ncol = (d.iec - d.isc + 3) * (d.jec - d.jsc + 3);
-  hipLaunchKernelGGL((update_dz_c_kernel<double>), dim3((ncol + 255) / 256, d.ntiles), dim3(256),
-                     0, s, d, dp_ref, zs, area, ut, vt, gz, work, ws, 1.0 / dt);
+  hipLaunchKernelGGL((update_dz_c_kernel<double, U>), dim3((ncol + 255) / 256, d.ntiles),
+                     dim3(256), 0, s, d, dp_ref, zs, area, ut, vt, gz, work, ws, 1.0 / dt);
   FV3_HIP_CHECK_LAUNCH(who);
   const Box bx{d.isc - 1, d.iec + 1, d.jsc - 1, d.jec + 1};
   const double* src = work;
@@ -719,13 +720,14 @@ int32_t fv3_update_dz_c_f64(const fv3_domain* dom, const double* area, const dou
   return FV3_OK;
 }
 
-int32_t fv3_update_dz_d_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t hord_tm,
-                            const int32_t nord4[4], const double* damp, const double* gk,
-                            const double* beta, const double* gamma, const double* zs,
-                            double* height, const double* crx, const double* cry,
-                            const double* xfx, const double* yfx, double* ws, double dt,
-                            double* work, int64_t work_slab, void* stream) {
-  using namespace fv3;
+// UpdateHeightOnDGrid (updatedzd.py) with the Courant numbers and area
+// fluxes of storage type U.
+template <typename U>
+int update_dz_d(const fv3_domain* dom, const fv3_metrics* g, int32_t hord_tm,
+                const int32_t nord4[4], const double* damp, const double* gk,
+                const double* beta, const double* gamma, const double* zs, double* height,
+                const U* crx, const U* cry, const U* xfx, const U* yfx, double* ws, double dt,
+                double* work, int64_t work_slab, void* stream) {
   const char* who = "fv3_update_dz_d";
   int rc = check_domain(dom, who);
   if (rc) return rc;
@@ -748,8 +750,9 @@ int32_t fv3_update_dz_d_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t
   double *crx_i = W[0], *xfx_i = W[1], *cry_i = W[2], *yfx_i = W[3], *fx = W[4], *fy = W[5];
   double *gxd = W[6], *gyd = W[7], *qy = W[8];
   const int ncol = (d.iec - d.isc + 7) * (d.jec - d.jsc + 7);
-  hipLaunchKernelGGL((spline_kernel<double>), dim3((ncol + 255) / 256, d.ntiles, 4), dim3(256), 0,
-                     s, d, gk, beta, gamma, crx, xfx, cry, yfx, crx_i, xfx_i, cry_i, yfx_i);
+  hipLaunchKernelGGL((spline_kernel<double, U>), dim3((ncol + 255) / 256, d.ntiles, 4),
+                     dim3(256), 0, s, d, gk, beta, gamma, crx, xfx, cry, yfx, crx_i, xfx_i,
+                     cry_i, yfx_i);
   FV3_HIP_CHECK_LAUNCH(who);
   const double* area = static_cast<const double*>(g->area);
   rc = fvtp2d_corner_fill_f64(dom, hord_tm, height, qy, crx_i, cry_i, xfx_i, yfx_i, nullptr,
@@ -765,6 +768,46 @@ int32_t fv3_update_dz_d_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t
                      s, d, area, zs, height, fx, fy, xfx_i, yfx_i, gxd, gyd, ws, dt);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
+}
+
+}  // namespace fv3
+
+extern "C" {
+
+int32_t fv3_update_dz_c_f64(const fv3_domain* dom, const double* area, const double* zs,
+                            const double* dp_ref, const double* ut, const double* vt,
+                            double* gz, double* ws, double dt, double* work, int64_t work_slab,
+                            void* stream) {
+  return fv3::update_dz_c<double>(dom, area, zs, dp_ref, ut, vt, gz, ws, dt, work, work_slab,
+                                  stream);
+}
+
+int32_t fv3_update_dz_c_mixed(const fv3_domain* dom, const double* area, const double* zs,
+                              const double* dp_ref, const float* ut, const float* vt,
+                              double* gz, double* ws, double dt, double* work,
+                              int64_t work_slab, void* stream) {
+  return fv3::update_dz_c<float>(dom, area, zs, dp_ref, ut, vt, gz, ws, dt, work, work_slab,
+                                 stream);
+}
+
+int32_t fv3_update_dz_d_f64(const fv3_domain* dom, const fv3_metrics* g, int32_t hord_tm,
+                            const int32_t nord4[4], const double* damp, const double* gk,
+                            const double* beta, const double* gamma, const double* zs,
+                            double* height, const double* crx, const double* cry,
+                            const double* xfx, const double* yfx, double* ws, double dt,
+                            double* work, int64_t work_slab, void* stream) {
+  return fv3::update_dz_d<double>(dom, g, hord_tm, nord4, damp, gk, beta, gamma, zs, height, crx,
+                                  cry, xfx, yfx, ws, dt, work, work_slab, stream);
+}
+
+int32_t fv3_update_dz_d_mixed(const fv3_domain* dom, const fv3_metrics* g, int32_t hord_tm,
+                              const int32_t nord4[4], const double* damp, const double* gk,
+                              const double* beta, const double* gamma, const double* zs,
+                              double* height, const float* crx, const float* cry,
+                              const float* xfx, const float* yfx, double* ws, double dt,
+                              double* work, int64_t work_slab, void* stream) {
+  return fv3::update_dz_d<float>(dom, g, hord_tm, nord4, damp, gk, beta, gamma, zs, height, crx,
+                                 cry, xfx, yfx, ws, dt, work, work_slab, stream);
 }
 
 int32_t fv3_nh_p_grad_f64(const fv3_domain* dom, const fv3_metrics* g, double* u, double* v,

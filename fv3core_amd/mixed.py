@@ -4,8 +4,8 @@ The reference computes in FP64 only (fv3core/utils/typing.py:24-25).  In the
 mixed form the bulk horizontal flux and advection stencils (c_sw, d_sw with
 its fv_tp_2d transports, del-n damping, divergence damping and a2b_ord4,
 Rayleigh damping, del2cubed, the halo exchange of their fields) hold and
-compute their fields in FP32, and FP64 is kept where the north-star
-extension keeps it: the Riemann solvers' tridiagonal pressure / geopotential
+compute their fields in FP32, and FP64 is kept for the Riemann solvers'
+tridiagonal pressure / geopotential
 solve, the pressure-edge and pk / pe integration (pe, peln, pk, pk3, pkc,
 pkz), the heights the solvers integrate (zh, gz, delz) and the pressure
 gradient on them.
@@ -15,8 +15,10 @@ gradient on them.
 * The FP64 solvers take their FP32 inputs directly (``fv3_riem_solver_c_mixed``,
   ``fv3_riem_solver3_mixed``, ``fv3_p_grad_c_mixed``, ``fv3_nh_p_grad_mixed``,
   ``fv3_pkz_tempadjust_mixed``: FP32 loads widened to FP64, FP64 results
-  rounded on store); the C- and D-grid height updates and the pressure
-  halos read FP64 copies made by ``fv3_cast_f32_to_f64``.
+  rounded on store), and so do the C- and D-grid height updates
+  (``fv3_update_dz_c_mixed``, ``fv3_update_dz_d_mixed``: FP32 winds, Courant
+  numbers and area fluxes widened exactly on load); the pressure halos and
+  nh_p_grad read an FP64 copy of delp made by ``fv3_cast_f32_to_f64``.
 """
 from __future__ import annotations
 

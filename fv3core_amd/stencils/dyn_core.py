@@ -114,12 +114,12 @@ class AcousticDynamics:
         for name in ("gz", "zh", "pem", "pkc", "pk3"):
             self._temporaries[name] = utils.zeros(shape3, dtype=dt_, device=dev)
         self._temporaries["ws3"] = utils.zeros(shape2, dtype=dt_, device=dev, batch_dims=len(lead))
-        # FP64 copies of FP32 fields for the FP64 kernels that read them
-        # through a copy (height updates, pressure halos, nh_p_grad's delp)
+        # FP64 copy of delp for the FP64 kernels that read it through a copy
+        # (the pressure halos and nh_p_grad); the height updates read the FP32
+        # winds, Courant numbers and area fluxes directly
         self._f64 = {}
         if self._mixed:
-            for name in ("ut", "vt", "crx", "cry", "xfx", "yfx", "delp"):
-                self._f64[name] = utils.zeros(shape3, dtype=dt_, device=dev)
+            self._f64["delp"] = utils.zeros(shape3, dtype=dt_, device=dev)
         grid_bulk = _mixed.grid_fp32(grid_data) if self._mixed else grid_data
         damp_bulk = (DampingCoefficients(grid_bulk.del6_u, grid_bulk.del6_v,
                                          damping_coefficients.da_min,
@@ -277,8 +277,7 @@ class AcousticDynamics:
                 else:
                     self._column(_OP_COPY, npz + 1, state.gz, a=state.zh)
                 self.update_geopotential_height_on_c_grid(
-                    self._dp_ref, self._zs, as64("ut", state.ut), as64("vt", state.vt), state.gz,
-                    state.ws3, dt2)
+                    self._dp_ref, self._zs, state.ut, state.vt, state.gz, state.ws3, dt2)
                 self.riem_solver_c(dt2, state.cappa, state.ptop, state.phis, state.ws3,
                                    state.ptc, state.q_con, state.delpc, state.gz, state.pkc,
                                    state.omga)
@@ -291,9 +290,8 @@ class AcousticDynamics:
                 state.heat_source, state.diss_estd, dt)
             self._h_scalar.update([state.delp, state.pt, state.q_con])
             if not hydro:
-                self.update_height_on_d_grid(self._zs, state.zh, as64("crx", state.crx),
-                                             as64("cry", state.cry), as64("xfx", state.xfx),
-                                             as64("yfx", state.yfx), state.wsd, dt)
+                self.update_height_on_d_grid(self._zs, state.zh, state.crx, state.cry,
+                                             state.xfx, state.yfx, state.wsd, dt)
                 self.riem_solver3(remap_step, dt, state.cappa, state.ptop, self._zs, state.wsd,
                                   state.delz, state.q_con, state.delp, state.pt, state.zh,
                                   state.pe, state.pkc, state.pk3, state.pk, state.peln,

@@ -42,8 +42,12 @@ class UpdateGeopotentialHeightOnCGrid:
         d = _lib.make_domain(self._gi, gz, nk, self._area, others3=(ut, vt), others2=(zs, ws))
         dp = self._column(dp_ref, gz.device)
         work, slab = self._work.get(gz, nk)
+        # ut, vt are FP64, or the mixed-precision step's FP32 storages
+        if ut.dtype != vt.dtype or ut.dtype not in (torch.float64, torch.float32):
+            raise TypeError("ut and vt must share one dtype, FP64 or FP32")
+        name = "fv3_update_dz_c_f64" if ut.dtype == torch.float64 else "fv3_update_dz_c_mixed"
         _lib.check(
-            _lib.lib().fv3_update_dz_c_f64(
+            getattr(_lib.lib(), name)(
                 ctypes.byref(d), _lib.ptr(self._area), _lib.ptr(zs), _lib.ptr(dp), _lib.ptr(ut),
                 _lib.ptr(vt), _lib.ptr(gz), _lib.ptr(ws), ctypes.c_double(dt), _lib.ptr(work),
                 ctypes.c_int64(slab), ctypes.c_void_p(_lib.stream_handle()),

@@ -79,8 +79,15 @@ class UpdateHeightOnDGrid:
                                       y_area_flux), others2=(surface_height, ws))
         c = self._cols(height.device)
         work, slab = self._work.get(height, self._nk)
+        # the Courant numbers and area fluxes are FP64, or the mixed-precision
+        # step's FP32 storages
+        bulk = courant_number_x.dtype
+        if bulk not in (torch.float64, torch.float32) or any(
+                t.dtype != bulk for t in (courant_number_y, x_area_flux, y_area_flux)):
+            raise TypeError("crx, cry, xfx and yfx must share one dtype, FP64 or FP32")
+        name = "fv3_update_dz_d_f64" if bulk == torch.float64 else "fv3_update_dz_d_mixed"
         _lib.check(
-            _lib.lib().fv3_update_dz_d_f64(
+            getattr(_lib.lib(), name)(
                 ctypes.byref(d), ctypes.byref(met), ctypes.c_int32(self._hord), self._nord4,
                 _lib.ptr(c["damp"]), _lib.ptr(c["gk"]), _lib.ptr(c["beta"]),
                 _lib.ptr(c["gamma"]), _lib.ptr(surface_height), _lib.ptr(height),

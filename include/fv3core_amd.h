@@ -482,6 +482,12 @@ int32_t fv3_update_dz_c_f64(const fv3_domain* d, const double* area, const doubl
                             const double* dp_ref, const double* ut, const double* vt,
                             double* gz, double* ws, double dt, double* work, int64_t work_slab,
                             void* stream);
+/* The same with FP32 ut, vt (the mixed-precision acoustic step's storages,
+ * widened exactly on load; gz, ws and the arithmetic stay FP64). */
+int32_t fv3_update_dz_c_mixed(const fv3_domain* d, const double* area, const double* zs,
+                              const double* dp_ref, const float* ut, const float* vt,
+                              double* gz, double* ws, double dt, double* work,
+                              int64_t work_slab, void* stream);
 
 /*
  * UpdateHeightOnDGrid.__call__ (fv3core/stencils/updatedzd.py:323-393) on
@@ -497,6 +503,14 @@ int32_t fv3_update_dz_d_f64(const fv3_domain* d, const fv3_metrics* g, int32_t h
                             double* height, const double* crx, const double* cry,
                             const double* xfx, const double* yfx, double* ws, double dt,
                             double* work, int64_t work_slab, void* stream);
+/* The same with FP32 crx, cry, xfx, yfx (widened exactly on load; the
+ * metric terms g, height and the arithmetic stay FP64). */
+int32_t fv3_update_dz_d_mixed(const fv3_domain* d, const fv3_metrics* g, int32_t hord_tm,
+                              const int32_t nord4[4], const double* damp, const double* gk,
+                              const double* beta, const double* gamma, const double* zs,
+                              double* height, const float* crx, const float* cry,
+                              const float* xfx, const float* yfx, double* ws, double dt,
+                              double* work, int64_t work_slab, void* stream);
 
 /*
  * NonHydrostaticPressureGradient.__call__ (fv3core/stencils/nh_p_grad.py:147-206)

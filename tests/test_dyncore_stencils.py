@@ -209,6 +209,55 @@ def test_gpu_update_dz_c_and_d(cube_tiles):
 
 
 @pytest.mark.gpu
+def test_gpu_update_dz_mixed_inputs_equal_the_fp64_entry():
+    """fv3_update_dz_c_mixed / fv3_update_dz_d_mixed take FP32 winds, Courant
+    numbers and area fluxes and widen them on load: bit for bit the FP64
+    entries' results on FP64 copies of the same FP32 values (the copies the
+    mixed-precision step used to make)."""
+    from fv3core_amd.stencils.updatedzc import UpdateGeopotentialHeightOnCGrid
+    from fv3core_amd.stencils.updatedzd import UpdateHeightOnDGrid
+    from fv3core_amd.utils.grid import DampingCoefficients
+
+    case = Case(16, NZ, True, seed=52)
+    grid = case.grid_data()
+    zh = _zh_profile(case, NZ)
+    zs = [np.full(case.shape[:2], 30.0) for _ in range(case.ntiles)]
+    ws = [np.zeros(case.shape[:2]) for _ in range(case.ntiles)]
+    dp0 = _dp0(NZ + 1)
+    ut, vt = case.field(0.0, 2e7), case.field(0.0, 2e7)
+    crx, cry = case.field(0.0, 0.15), case.field(0.0, 0.15)
+    xfx = [c * m["dy"][..., None] * 2e4 for c, m in zip(crx, case.metric)]
+    yfx = [c * m["dx"][..., None] * 2e4 for c, m in zip(cry, case.metric)]
+    col = {"damp_vt": np.full(NZ + 1, 0.06), "nord_v": np.array([0, 0, 2] + [2] * (NZ - 2))}
+    damp = DampingCoefficients(grid.del6_u, grid.del6_v, case.scalars["da_min"],
+                               case.scalars["da_min_c"])
+
+    def pair(arrs):
+        f64 = _dev(case, arrs)
+        f32 = torch.empty_like(f64, dtype=torch.float32)
+        f32.copy_(f64)
+        f64.copy_(f32)  # the FP64 copy holds exactly the FP32 values
+        return f32, f64
+
+    winds = [pair(a) for a in (ut, vt)]
+    fluxes = [pair(a) for a in (crx, cry, xfx, yfx)]
+    out = []
+    for form in (0, 1):
+        gz_d, ws_d = _dev(case, zh), _dev2(case, ws)
+        UpdateGeopotentialHeightOnCGrid(case.factory, grid.area)(
+            dp0, _dev2(case, zs), winds[0][form], winds[1][form], gz_d, ws_d, 18.75)
+        h_d, ws2_d = _dev(case, zh), _dev2(case, ws)
+        UpdateHeightOnDGrid(case.factory, damp, grid, 0, 6, dp0, col,
+                            [[0, 1], [1, 1], [2, 1], [3, 1]])(
+            _dev2(case, zs), h_d, *(f[form] for f in fluxes), ws2_d, 37.5)
+        out.append((gz_d, ws_d, h_d, ws2_d))
+    torch.cuda.synchronize()
+    assert winds[0][0].dtype == torch.float32
+    for a, b in zip(*out):
+        torch.testing.assert_close(a, b, rtol=0, atol=0)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("cube_tiles", [False, True])
 def test_gpu_nh_p_grad_and_halos(cube_tiles):
     from fv3core_amd.stencils.nh_p_grad import NonHydrostaticPressureGradient
