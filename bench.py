@@ -611,20 +611,27 @@ def _time_ensemble(parts, steps, warmup, roofline_kernel, graph):
     wall = time.perf_counter() - t0
     cnt, kern_ms, kern_bytes = _timing(lib, roofline_kernel)
     lib.fv3_timing_arm(None)
-    graph_wall = None
-    if graph:  # every batch in one HIP graph, one launch per call (eager, capture, replays)
-        gens = UnevenEnsemble(parts, use_graph=True)
-        for _ in range(3):
-            gens()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            gens()
-        torch.cuda.synchronize()
-        graph_wall = time.perf_counter() - t0
+    graph_wall = graph_seq_wall = None
+    if graph:
+        # every batch in one HIP graph, one launch per call (eager, capture,
+        # replays): the small batches as parallel branches beside the large
+        # one (the value), and in sequence
+        walls = []
+        for concurrent in (True, False):
+            gens = UnevenEnsemble(parts, use_graph=True, concurrent=concurrent)
+            for _ in range(3):
+                gens()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                gens()
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+            del gens
+        graph_wall, graph_seq_wall = walls
     nz = parts[0][1].pt.shape[-1] - 1
     finite = all(bool(torch.isfinite(st.pt[:, 3:-4, 3:-4, :nz]).all().item()) for _, st in parts)
-    return wall, graph_wall, cnt, kern_ms, kern_bytes, finite
+    return wall, (graph_wall, graph_seq_wall), cnt, kern_ms, kern_bytes, finite
 
 
 def _ensemble_mixed_errors(cfg, n=96, pt_sd=0.0, uv_sd=0.0, seed=96000):
@@ -699,8 +706,8 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
         del parts
         gc.collect()
         torch.cuda.empty_cache()
-    wall64, graph64, _, _, _, finite64 = results["fp64"]
-    wall, graph_wall, cnt, kern_ms, kern_bytes, finite = results["mixed"]
+    wall64, (graph64, graph64_seq), _, _, _, finite64 = results["fp64"]
+    wall, (graph_wall, graph_seq), cnt, kern_ms, kern_bytes, finite = results["mixed"]
     used = graph_wall if graph_wall else wall
     used64 = graph64 if graph64 else wall64
     avg_s = kern_ms / max(cnt, 1) * 1e-3
@@ -711,6 +718,8 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
         "ms_per_step": used / steps * 1e3,
         "eager_ms_per_step": wall / steps * 1e3,
         "graph_ms_per_step": graph_wall / steps * 1e3 if graph_wall else None,
+        "graph_sequential_ms_per_step": graph_seq / steps * 1e3 if graph_seq else None,
+        "fp64_graph_sequential_ms_per_step": graph64_seq / steps * 1e3 if graph64_seq else None,
         "fp64_value": steps * cells / used64,
         "fp64_ms_per_step": used64 / steps * 1e3,
         "fp64_eager_ms_per_step": wall64 / steps * 1e3,
@@ -730,7 +739,9 @@ def bench_ensemble(steps: int, warmup: int, graph: bool, roofline_kernel: str,
             "workload": "batched ensemble: 32 members x 6 x C48 x 79 in one batch + uneven "
             "members C24/C64/C96/C128 (one cube each), AcousticDynamics n_split=6, dt=225 s, "
             "perturbed jet states; mixed form: " + MIXED_FORM +
-            (", every batch of a call in one HIP graph (one launch per call)" if graph else ""),
+            (", every batch of a call in one HIP graph (one launch per call), the four "
+             "single-cube batches as parallel branches beside the 32-member batch" if graph
+             else ""),
             "members": 36, "tiles": 6 * 36, "nz": nz, "n_split": n_split,
             "cells_per_step": cells,
             "state_finite_after_timed_steps": bool(finite and finite64),
@@ -910,7 +921,8 @@ def main():
                   "mixed_over_fp64", "step_roofline", "adjoint_over_forward",
                   "adjoint_workload", "reverse_ms_per_step", "mixed_vs_fp64", "mixed_gate_ok",
                   "mixed_vs_fp64_perturbed_c48", "mixed_gate_ok_perturbed_c48",
-                  "fp64_eager_ms_per_step"):
+                  "fp64_eager_ms_per_step", "graph_sequential_ms_per_step",
+                  "fp64_graph_sequential_ms_per_step"):
         if res.get(extra) is not None:
             line[extra] = res[extra]
     if res.get("copy_calibration_bytes"):

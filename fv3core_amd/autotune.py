@@ -9,10 +9,12 @@ tuner picks the fastest on the GPU it runs on:
    shape's ``-D`` flags and link it with the other objects of the in-tree
    build into ``_tune/lib_<kernel>_<a>_<b>.so`` (CPU only: hipcc
    cross-compiles for gfx950).
-2. ``measure``: run the acoustic-step benchmark once per library, each in
-   its own process, timing the kernel with the library's HIP-event hooks
+2. ``measure``: run a benchmark workload once per library, each in its own
+   process, timing the kernel with the library's HIP-event hooks
    (``bench.py --roofline-kernel <kernel>``), and write the mean launch
-   times to ``_tune/results.json``.
+   times to ``_tune/results.json``.  ``--workload ensemble`` (the default)
+   times the launches of the 32-member C48 batch in the mixed form;
+   ``--workload dyncore`` those of the FP64 acoustic step on six C96 tiles.
 3. ``apply``: rewrite the defaults in ``tile_config.hpp`` with the fastest
    shape of each kernel (rebuild the library afterwards).
 
@@ -83,14 +85,21 @@ import pathlib, runpy, sys
 sys.path.insert(0, {repo!r})
 from fv3core_amd import _lib
 _lib.LIB_PATH = pathlib.Path({lib!r})
-sys.argv = ["bench.py", "--steps", "{steps}", "--warmup", "1", "--no-cpu-baseline",
-            "--roofline-kernel", {name!r}]
+sys.argv = ["bench.py", "--workload", {workload!r}, "--steps", "{steps}", "--warmup", "1",
+            "--no-cpu-baseline", "--roofline-kernel", {name!r}] + {extra!r}
 runpy.run_path({bench!r}, run_name="__main__")
 """
 
 
-def measure(kernels=None, steps: int = 3, timeout: int = 300) -> dict:
-    """Mean launch time (us) of each variant's kernel inside the acoustic step."""
+# bench.py options that leave out what the kernel timing does not need
+_WORKLOAD_ARGS = {"ensemble": ["--no-graph", "--no-adjoint"], "dyncore": []}
+
+
+def measure(kernels=None, steps: int = 3, timeout: int = 300,
+            workload: str = "ensemble") -> dict:
+    """Mean launch time (us) of each variant's kernel inside the workload."""
+    if workload not in _WORKLOAD_ARGS:
+        raise ValueError(f"unknown workload {workload}")
     results = {}
     for kernel in kernels or KERNELS:
         _, name, _, shapes = KERNELS[kernel]
@@ -99,7 +108,8 @@ def measure(kernels=None, steps: int = 3, timeout: int = 300) -> dict:
             if not lib.exists():
                 raise FileNotFoundError(f"{lib} is missing: run the build step first")
             code = _RUNNER.format(repo=str(REPO), lib=str(lib), steps=steps, name=name,
-                                  bench=str(REPO / "bench.py"))
+                                  bench=str(REPO / "bench.py"), workload=workload,
+                                  extra=_WORKLOAD_ARGS[workload])
             res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
                                  timeout=timeout)
             if res.returncode != 0:
@@ -145,12 +155,13 @@ def main(argv=None) -> int:
     p.add_argument("step", choices=["build", "measure", "apply"])
     p.add_argument("--kernel", action="append", choices=sorted(KERNELS))
     p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--workload", default="ensemble", choices=sorted(_WORKLOAD_ARGS))
     a = p.parse_args(argv)
     if a.step == "build":
         for lib in build_variants(a.kernel):
             print(lib)
     elif a.step == "measure":
-        measure(a.kernel, a.steps)
+        measure(a.kernel, a.steps, workload=a.workload)
     else:
         for kernel, shape in apply().items():
             print(f"{kernel}: rows={shape[0]} waves={shape[1]}")

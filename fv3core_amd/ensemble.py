@@ -114,24 +114,47 @@ def member_slice(state, m: int):
 
 class UnevenEnsemble:
     """Members of several resolutions: one batched AcousticDynamics per
-    resolution, called in sequence by :meth:`__call__`.
+    resolution, called by :meth:`__call__`.
 
     With ``use_graph=True`` every resolution's call is captured into one HIP
     graph, so that a call of the whole ensemble (all its acoustic substeps,
     halo exchanges and components, for every batch) is one graph launch: the
     first call runs eagerly (it fills the component caches), the second
     captures and replays, later calls replay.  The parts' own graph option
-    must be off; the storages must stay the same between calls."""
+    must be off; the storages must stay the same between calls.
 
-    def __init__(self, parts: Sequence, use_graph: bool = False):
+    With ``concurrent=True`` the first part runs on the caller's stream and
+    every other part on a stream of its own (default priority), forked from
+    and joined back into the caller's stream, so that the small batches,
+    whose launches cannot fill the GPU, run beside the large one; in a
+    captured graph they are parallel branches.  The parts share no storage,
+    so the results are those of the sequential call bit for bit."""
+
+    def __init__(self, parts: Sequence, use_graph: bool = False, concurrent: bool = False):
         self.parts: List = list(parts)  # (acoustic, state) pairs
         self._use_graph = bool(use_graph)
+        self._concurrent = bool(concurrent)
+        self._streams = None
         self._calls = 0
         self.graph = None
 
     def _run(self):
-        for acoustic, state in self.parts:
-            acoustic(state)
+        if not self._concurrent or len(self.parts) < 2:
+            for acoustic, state in self.parts:
+                acoustic(state)
+            return
+        main = torch.cuda.current_stream()
+        if self._streams is None:
+            self._streams = [torch.cuda.Stream(device=main.device) for _ in self.parts[1:]]
+        for s in self._streams:
+            s.wait_stream(main)
+        acoustic, state = self.parts[0]
+        acoustic(state)
+        for s, (acoustic, state) in zip(self._streams, self.parts[1:]):
+            with torch.cuda.stream(s):
+                acoustic(state)
+        for s in self._streams:
+            main.wait_stream(s)
 
     def __call__(self):
         if not self._use_graph:

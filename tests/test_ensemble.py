@@ -146,29 +146,35 @@ def test_gpu_uneven_ensemble_members_match_oracle(mixed):
 
 
 @pytest.mark.gpu
-def test_gpu_uneven_ensemble_one_graph_equals_eager():
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_gpu_uneven_ensemble_one_graph_equals_eager(concurrent):
     """UnevenEnsemble(use_graph=True): one HIP graph for the calls of every
-    resolution; each replay equals the eager ensemble bit for bit, with
-    unrelated eager work and host synchronisations between the replays."""
+    resolution; each replay equals the sequential eager ensemble bit for bit,
+    with unrelated eager work and host synchronisations between the replays.
+    With concurrent=True the second part runs on its own stream (eagerly and
+    as a parallel branch of the graph) and the results are the same bits."""
     from fv3core_amd.ensemble import UnevenEnsemble, ensemble_acoustic
 
     cfg = DynamicalCoreConfig(npz=NZ, n_split=2, k_split=1, nord=3, dt_atmos=225)
     runs = []
-    for use_graph in (False, True):
+    for form in ("eager", "eager_c", "graph"):
         parts = []
         for n, members in ((12, 2), (16, 1)):
             init = cube_state(n, NZ, seed=6)
             parts.append(ensemble_acoustic(init, members, cfg, seed=50 + n))
-        runs.append((UnevenEnsemble(parts, use_graph=use_graph), parts))
+        runs.append((UnevenEnsemble(parts, use_graph=form == "graph",
+                                    concurrent=concurrent and form != "eager"), parts))
     scratch = torch.zeros(1 << 16, dtype=torch.float64, device="cuda")
     for call in range(4):
-        runs[0][0]()
-        runs[1][0]()
+        for ens, _ in runs:
+            ens()
         scratch.add_(1.0)
         torch.cuda.synchronize()
-        for (_, sa), (_, sb) in zip(runs[0][1], runs[1][1]):
-            for k in FIELDS:
-                torch.testing.assert_close(getattr(sb, k), getattr(sa, k), rtol=0, atol=0,
-                                           equal_nan=True,
-                                           msg=f"call {call}: {k} differs between graph and eager")
-    assert runs[1][0].graph is not None
+        for other in (1, 2):
+            for (_, sa), (_, sb) in zip(runs[0][1], runs[other][1]):
+                for k in FIELDS:
+                    torch.testing.assert_close(
+                        getattr(sb, k), getattr(sa, k), rtol=0, atol=0, equal_nan=True,
+                        msg=f"call {call}: {k} differs between form {other} and eager")
+    assert runs[2][0].graph is not None
+
