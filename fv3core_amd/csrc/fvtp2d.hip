@@ -15,7 +15,8 @@ namespace fv3 {
 // Wave-strip march.  One wave (64 lanes) owns a strip of 64 columns of one
 // level of one tile: 58 output interfaces in i plus a 3-cell halo on each
 // side, lane L <-> column i0 - 3 + L.  It marches up the rows of a segment of
-// MTJ output rows.  Stencils across i read the neighbouring lanes' values
+// SEG output rows (MTJ or MTJ_TALL, chosen per launch; SEG + 6 march steps,
+// a multiple of the unroll factor 3).  Stencils across i read the neighbouring lanes' values
 // with DPP lane shifts; stencils along j keep a sliding window of rows in
 // registers.  Each PPM face value al and each cell's edge perturbations
 // (bl, br and the smoothness flag, or the monotone slope dm for hord 8) are
@@ -24,7 +25,8 @@ namespace fv3 {
 // last bit.  No LDS and no barriers: waves run independently.
 constexpr int MW = 64;        // lanes = strip columns
 constexpr int MTI = MW - 6;   // output interfaces per strip in i
-constexpr int MTJ = 33;       // output rows per segment in j
+constexpr int MTJ = 33;       // output rows per segment in j (default)
+constexpr int MTJ_TALL = 51;  // the taller segment of large batches of narrow tiles
 
 // Whole-wave shifts by one lane with DPP: wave_shr:1 (lane L reads lane
 // L - 1) and wave_shl:1 (lane L reads lane L + 1); the lane without a source
@@ -64,6 +66,7 @@ __device__ __forceinline__ T lane_up(T v, int d) {  // value of lane - d
 // Number of XCDs on MI355X: consecutive workgroups are dispatched round-robin
 // over them, each with its own L2.
 constexpr int kNumXcd = 8;
+constexpr int kSimdsPerGpu = 256 * 4;  // 256 CUs of 4 SIMDs
 
 // Workgroup id -> logical unit so that each XCD runs a contiguous range of
 // logical units (whole planes of one level), keeping the halo rows that
@@ -276,7 +279,7 @@ __device__ __forceinline__ P pick_slot(const P (&a)[kMaxScalarBatch], int b) {
 
 // S is the storage type of every field, T the type the march computes in
 // (the same, or float for FP64 storage in the mixed-precision form).
-template <typename S, typename T, int ORD, bool EX, bool EY>
+template <typename S, typename T, int ORD, bool EX, bool EY, int SEG>
 __device__ __forceinline__ void fvtp2d_strip(
     const Dom& d, const YFill& yf, int i0, int j0, const S* __restrict__ qx,
     const S* __restrict__ qy,
@@ -379,13 +382,13 @@ __device__ __forceinline__ void fvtp2d_strip(
   load_step(j0 - 3, nx);
   T ay_g = T(0);  // ay of row g = r - 3 (row f of the previous step)
 #pragma unroll 3
-  for (int s = 0; s < MTJ + 6; ++s) {
+  for (int s = 0; s < SEG + 6; ++s) {
     const int r = j0 - 3 + s;
     const StepIn cu = nx;
     load_step(r + 1, nx);
     const int f = r - 2, g = r - 3;
-    const bool wfx = out_lane && g >= j0 && g < j0 + MTJ && i <= d.iec + 1 && g <= d.jec;
-    const bool wfy = out_lane && f >= j0 && f < j0 + MTJ && i <= d.iec && f <= d.jec + 1;
+    const bool wfx = out_lane && g >= j0 && g < j0 + SEG && i <= d.iec + 1 && g <= d.jec;
+    const bool wfy = out_lane && f >= j0 && f < j0 + SEG && i <= d.iec && f <= d.jec + 1;
     // x-inner PPM of q_x on row r (faces i), then q_j of row r
     // (fvtp2d.py:50-60)
     const T xm = ppm_row<T, ORD, EX>(cu.qx, cu.cx, i, d.is, d.ie, cu.dx, dx + jrow(r));
@@ -415,7 +418,7 @@ __device__ __forceinline__ void fvtp2d_strip(
   }
 }
 
-template <typename S, typename T, int ORD>
+template <typename S, typename T, int ORD, int SEG>
 __global__ void __launch_bounds__(MW)
     fvtp2d_kernel(Dom d, ScalarBatch<S> sb, const S* __restrict__ crx,
                   const S* __restrict__ cry, const S* __restrict__ xfx,
@@ -437,19 +440,19 @@ __global__ void __launch_bounds__(MW)
   S* __restrict__ fx = pick_slot(sb.fx, b);
   S* __restrict__ fy = pick_slot(sb.fy, b);
   const int i0 = d.isc + bi * MTI;
-  const int j0 = d.jsc + bj * MTJ;
+  const int j0 = d.jsc + bj * SEG;
   const int64_t off3 = (int64_t)t * d.st + (int64_t)k * d.sk;
   const int64_t off2 = (int64_t)t * d.st2;
   const S* ux = (xmf ? xmf : xfx) + off3;
   const S* uy = (ymf ? ymf : yfx) + off3;
   // the tile-edge formulas of a direction are needed when the strip's
-  // stencil reach (cells i0-4 .. i0+MTI+4, rows j0-4 .. j0+MTJ+4) touches
+  // stencil reach (cells i0-4 .. i0+MTI+4, rows j0-4 .. j0+SEG+4) touches
   // an edge region of that direction
   auto near = [](int lo, int hi, int a) { return hi >= a - 2 && lo <= a + 3; };
   const bool ex = near(i0 - 4, i0 + MTI + 4, d.is) || near(i0 - 4, i0 + MTI + 4, d.ie);
-  const bool ey = near(j0 - 4, j0 + MTJ + 4, d.js) || near(j0 - 4, j0 + MTJ + 4, d.je);
+  const bool ey = near(j0 - 4, j0 + SEG + 4, d.js) || near(j0 - 4, j0 + SEG + 4, d.je);
 #define FV3_TP_STRIP(EX_, EY_)                                                                  \
-  fvtp2d_strip<S, T, ORD, EX_, EY_>(d, yf, i0, j0, q_x + off3, q_y + off3, crx + off3,        \
+  fvtp2d_strip<S, T, ORD, EX_, EY_, SEG>(d, yf, i0, j0, q_x + off3, q_y + off3, crx + off3, \
                                  cry + off3,                                                  \
                                  xfx + off3, yfx + off3, ux, uy, area + off2, dxa + off2,     \
                                  dya + off2, fx + off3, fy + off3)
@@ -490,7 +493,22 @@ static int fvtp2d_batch_launch(const fv3_domain* dom, int hord, const ScalarBatc
               "%s: level too large", who);
   Dom d = to_dom(dom);
   const int nfi = d.iec + 2 - d.isc, nfj = d.jec + 2 - d.jsc;
-  const int nbi = (nfi + MTI - 1) / MTI, nbj = (nfj + MTJ - 1) / MTJ;
+  const int nbi = (nfi + MTI - 1) / MTI;
+  // Segment height: each segment marches its rows plus 6 halo rows, so
+  // taller segments take fewer steps per output row, but fewer waves.  The
+  // taller segment is taken when it needs fewer march steps and the launch
+  // still has four waves per SIMD (large batches of narrow tiles, such as
+  // 192 C48 tiles); otherwise 33 rows (six C96 tiles: 97 rows in three
+  // segments, the fastest height of a sweep over 13-49 rows).
+  const int64_t planes = (int64_t)nbi * d.nk * d.ntiles * sb.n;
+  int mtj = MTJ;
+  {
+    const int nb_s = (nfj + MTJ - 1) / MTJ, nb_t = (nfj + MTJ_TALL - 1) / MTJ_TALL;
+    if ((int64_t)nb_t * (MTJ_TALL + 6) < (int64_t)nb_s * (MTJ + 6) &&
+        planes * nb_t >= 4 * kSimdsPerGpu)
+      mtj = MTJ_TALL;
+  }
+  const int nbj = (nfj + mtj - 1) / mtj;
   const int64_t total = (int64_t)nbi * nbj * d.nk * d.ntiles * sb.n;
   FV3_REQUIRE(total < (int64_t)1 << 30, FV3_E_ARG, "%s: grid too large", who);
   const int64_t launched = (total + kNumXcd - 1) / kNumXcd * kNumXcd;
@@ -508,24 +526,31 @@ static int fvtp2d_batch_launch(const fv3_domain* dom, int hord, const ScalarBatc
   TimedLaunch timed("fvtp2d_kernel", s,
                     nfld * sizeof(T) * (double)(d.iec - d.isc + 1) * (d.jec - d.jsc + 1) *
                         d.nk * d.ntiles);
-  switch (hord) {
-    case 5:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 5>), grid, block, 0, s, d, sb, crx, cry, xfx, yfx,
-                         xmf, ymf, area, dxa, dya, nbi, nbj, yf);
-      break;
-    case 6:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 6>), grid, block, 0, s, d, sb, crx, cry, xfx, yfx,
-                         xmf, ymf, area, dxa, dya, nbi, nbj, yf);
-      break;
-    case 7:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 7>), grid, block, 0, s, d, sb, crx, cry, xfx, yfx,
-                         xmf, ymf, area, dxa, dya, nbi, nbj, yf);
-      break;
-    default:
-      hipLaunchKernelGGL((fvtp2d_kernel<T, C, 8>), grid, block, 0, s, d, sb, crx, cry, xfx, yfx,
-                         xmf, ymf, area, dxa, dya, nbi, nbj, yf);
-      break;
-  }
+  auto go = [&](auto seg) {
+    constexpr int SEG = decltype(seg)::value;
+    switch (hord) {
+      case 5:
+        hipLaunchKernelGGL((fvtp2d_kernel<T, C, 5, SEG>), grid, block, 0, s, d, sb, crx, cry, xfx,
+                           yfx, xmf, ymf, area, dxa, dya, nbi, nbj, yf);
+        break;
+      case 6:
+        hipLaunchKernelGGL((fvtp2d_kernel<T, C, 6, SEG>), grid, block, 0, s, d, sb, crx, cry, xfx,
+                           yfx, xmf, ymf, area, dxa, dya, nbi, nbj, yf);
+        break;
+      case 7:
+        hipLaunchKernelGGL((fvtp2d_kernel<T, C, 7, SEG>), grid, block, 0, s, d, sb, crx, cry, xfx,
+                           yfx, xmf, ymf, area, dxa, dya, nbi, nbj, yf);
+        break;
+      default:
+        hipLaunchKernelGGL((fvtp2d_kernel<T, C, 8, SEG>), grid, block, 0, s, d, sb, crx, cry, xfx,
+                           yfx, xmf, ymf, area, dxa, dya, nbi, nbj, yf);
+        break;
+    }
+  };
+  if (mtj == MTJ_TALL)
+    go(std::integral_constant<int, MTJ_TALL>{});
+  else
+    go(std::integral_constant<int, MTJ>{});
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
 }

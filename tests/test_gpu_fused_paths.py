@@ -3,6 +3,11 @@
 * fv3_fvtp2d_corner_fill: CopyCornersXY(q, q_x = q, q_y) + fv_tp_2d in one
   call (the y corner fill applied as the transport reads q) against the two
   calls, on domains with all four and with two cube corners.
+* fv_tp_2d segment height: the launcher takes 51-row segments for large
+  batches of narrow tiles and 33-row segments otherwise; columns of output
+  rows are computed the same way in either, so one call on 64 C48 tiles
+  (51-row segments) must equal eight calls on 8 tiles each (33-row
+  segments) bit for bit, FP64 and FP32.
 * map_single: the launcher takes the fused producer/walk form from 1024
   waves of columns up and the four-sweep form below; columns are
   independent, so one call on six 128 x 128 tiles (fused form) must equal
@@ -60,6 +65,47 @@ def test_corner_fill_transport_equals_copy_then_transport(hord, edges):
     assert torch.equal(q_a, q_b), "the x corner fill of q differs"
     assert torch.equal(fx_a, fx_b) and torch.equal(fy_a, fy_b), "fluxes differ"
     assert float(fx_b.abs().max()) > 0.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("hord", [6, 8])
+def test_fvtp2d_tall_segments_equal_regular_segments(dtype, hord):
+    n, nz, nt, chunk = 48, 79, 64, 8
+    gi = GridIndexing((n, n, nz), 3, True, True, True, True)
+    g = torch.Generator(device="cuda").manual_seed(4800 + hord)
+    shape = (nt, n + 7, n + 7, nz + 1)
+
+    def rand(lo, hi, shp=shape):
+        t = utils.empty(shp, dtype=dtype, batch_dims=1)
+        t.copy_(lo + (hi - lo) * torch.rand(shp, generator=g, device="cuda", dtype=torch.float64))
+        return t
+
+    q = rand(0.0, 1.0)
+    crx, cry = rand(-0.45, 0.45), rand(-0.45, 0.45)
+    xfx, yfx = rand(-2e9, 2e9), rand(-2e9, 2e9)
+    area = rand(4e10, 6e10, (nt, n + 7, n + 7))
+    dxa, dya = rand(1.8e5, 2.2e5, (nt, n + 7, n + 7)), rand(1.8e5, 2.2e5, (nt, n + 7, n + 7))
+    name = "fv3_fvtp2d_f64" if dtype == torch.float64 else "fv3_fvtp2d_f32"
+    fn = getattr(_lib.lib(), name)
+    s = ctypes.c_void_p(_lib.stream_handle())
+
+    def run(sl):
+        fx, fy = torch.zeros_like(q[sl]), torch.zeros_like(q[sl])
+        f3 = (q[sl], crx[sl], cry[sl], xfx[sl], yfx[sl], fx, fy)
+        dd = _lib.make_domain(gi, q[sl], nz, area[sl], others3=f3,
+                              others2=(dxa[sl], dya[sl]))
+        _lib.check(fn(ctypes.byref(dd), hord, _lib.ptr(q[sl]), _lib.ptr(q[sl]),
+                      _lib.ptr(crx[sl]), _lib.ptr(cry[sl]), _lib.ptr(xfx[sl]), _lib.ptr(yfx[sl]),
+                      None, None, _lib.ptr(area[sl]), _lib.ptr(dxa[sl]), _lib.ptr(dya[sl]),
+                      _lib.ptr(fx), _lib.ptr(fy), s))
+        return fx, fy
+
+    fx_all, fy_all = run(slice(0, nt))
+    for c in range(0, nt, chunk):
+        fx_c, fy_c = run(slice(c, c + chunk))
+        assert torch.equal(fx_all[c:c + chunk], fx_c), f"x fluxes of tiles {c}..{c + chunk - 1}"
+        assert torch.equal(fy_all[c:c + chunk], fy_c), f"y fluxes of tiles {c}..{c + chunk - 1}"
+    assert float(fx_all.abs().max()) > 0.0
 
 
 @pytest.mark.parametrize("kord,mode", [(9, 1), (10, 0), (10, -2), (9, -1)])
