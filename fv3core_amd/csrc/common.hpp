@@ -73,4 +73,19 @@ __device__ __forceinline__ T sgn(T a, T b) {
   return b > T(0) ? aa : -aa;
 }
 
+// Number of XCDs on MI355X: consecutive workgroups are dispatched round-robin
+// over them, each with its own L2.
+constexpr int kNumXcd = 8;
+constexpr int kSimdsPerGpu = 256 * 4;  // 256 CUs of 4 SIMDs
+
+// Workgroup id -> logical unit so that each XCD runs a contiguous range of
+// logical units (whole planes of one level, consecutive levels of one
+// tile), keeping what neighbouring units share (halo rows, a tile's metric
+// terms) in that XCD's L2.  A launch has a multiple of kNumXcd workgroups;
+// ids mapped at or past `total` have no unit.
+__device__ __forceinline__ int xcd_logical_block(int hw, int total) {
+  const int per = (total + kNumXcd - 1) / kNumXcd;
+  return (hw % kNumXcd) * per + hw / kNumXcd;
+}
+
 }  // namespace fv3

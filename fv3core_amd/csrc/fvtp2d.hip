@@ -63,18 +63,6 @@ __device__ __forceinline__ T lane_up(T v, int d) {  // value of lane - d
   return v;
 }
 
-// Number of XCDs on MI355X: consecutive workgroups are dispatched round-robin
-// over them, each with its own L2.
-constexpr int kNumXcd = 8;
-constexpr int kSimdsPerGpu = 256 * 4;  // 256 CUs of 4 SIMDs
-
-// Workgroup id -> logical unit so that each XCD runs a contiguous range of
-// logical units (whole planes of one level), keeping the halo rows that
-// neighbouring units share in that XCD's L2.
-__device__ __forceinline__ int xcd_logical_block(int hw, int total) {
-  const int per = (total + kNumXcd - 1) / kNumXcd;
-  return (hw % kNumXcd) * per + hw / kNumXcd;
-}
 
 // Face value al at face f (hord < 8, compute_al, xppm.py:147-181) from the
 // cells f-2, f-1, f, f+1 (qa..qd) and their widths (da..dd, read only at the
