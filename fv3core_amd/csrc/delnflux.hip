@@ -20,10 +20,13 @@ namespace fv3 {
 // y+4, ... so no index is ever divided.
 constexpr int DN_W = 64;            // LDS columns = TI + 6
 constexpr int DN_TI = DN_W - 6;     // interfaces per block in i
-constexpr int DN_TJ = FV3_DN_TJ;    // interfaces per block in j
-constexpr int DN_H = DN_TJ + 6;     // LDS rows
-constexpr int DN_NY = FV3_DN_NY;    // waves per block
-constexpr int DN_NT = DN_W * DN_NY;
+// Block shapes (interface rows, waves): the default, and a taller one for
+// FP32 launches where it computes fewer LDS rows (the 6 halo rows are paid
+// once per block); in FP64 its LDS tile halves the blocks a CU holds.
+constexpr int DN_TJ = FV3_DN_TJ;
+constexpr int DN_NY = FV3_DN_NY;
+constexpr int DN_TJ_TALL = FV3_DN_TJ_TALL;
+constexpr int DN_NY_TALL = FV3_DN_NY_TALL;
 
 struct Box {
   int i0, i1, j0, j1;  // inclusive absolute bounds
@@ -32,14 +35,15 @@ struct Box {
   }
 };
 
-template <typename T>
-__global__ void __launch_bounds__(DN_NT)
+template <typename T, int TJ_, int NY_>
+__global__ void __launch_bounds__(DN_W * NY_)
     delnflux_kernel(Dom d, int mode, int4 nord4, int nmax, CornerTable tab,
                     const T* __restrict__ q, const T* __restrict__ mass,
                     const T* __restrict__ damp, const T* __restrict__ del6_u,
                     const T* __restrict__ del6_v, const T* __restrict__ rarea,
                     T* __restrict__ fx, T* __restrict__ fy, int nbi) {
-  constexpr int W = DN_W, H = DN_H, TI = DN_TI, TJ = DN_TJ;
+  constexpr int W = DN_W, TI = DN_TI, TJ = TJ_, H = TJ_ + 6;  // LDS rows H
+  constexpr int DN_NY = NY_, DN_NT = DN_W * NY_;
   constexpr int NR = (H + DN_NY - 1) / DN_NY;  // LDS rows per lane
   __shared__ T sd2[H][W];
   __shared__ T sfx[H][W];
@@ -230,15 +234,29 @@ static int delnflux_launch(const fv3_domain* dom, int mode, const int32_t* nord4
   CornerTable tab;
   build_corner_table(dom, 3, &tab);
   const int nfi = d.iec + 2 - d.isc, nfj = d.jec + 2 - d.jsc;
-  const int nbi = (nfi + DN_TI - 1) / DN_TI, nbj = (nfj + DN_TJ - 1) / DN_TJ;
+  const int nbi = (nfi + DN_TI - 1) / DN_TI;
+  // the tall shape for FP32 when it computes at least a quarter fewer LDS
+  // rows per column of blocks (192 C48 tiles: 2 x 31 rows instead of 7 x 14,
+  // 602 -> 494 us per launch; in FP64 six C96 tiles keep the default, 86 us
+  // against 105 us tall)
+  const int nbj_d = (nfj + DN_TJ - 1) / DN_TJ, nbj_t = (nfj + DN_TJ_TALL - 1) / DN_TJ_TALL;
+  const bool tall = sizeof(T) == 4 &&
+                    4 * nbj_t * (DN_TJ_TALL + 6) <= 3 * nbj_d * (DN_TJ + 6);
+  const int nbj = tall ? nbj_t : nbj_d;
   dim3 grid(nbi * nbj, d.nk, d.ntiles);
   int4 n4 = make_int4(nord4[0], nord4[1], nord4[2], nord4[3]);
   // q read, fx / fy read and written (per cell-level of the compute domain)
   TimedLaunch timed("delnflux_kernel", as_stream(stream),
                     5.0 * sizeof(T) * (double)(d.iec - d.isc + 1) * (d.jec - d.jsc + 1) * d.nk *
                         d.ntiles);
-  hipLaunchKernelGGL((delnflux_kernel<T>), grid, dim3(DN_W, DN_NY), 0, as_stream(stream), d, mode, n4,
-                     nmax, tab, q, mass, damp, del6_u, del6_v, rarea, fx, fy, nbi);
+  if (tall)
+    hipLaunchKernelGGL((delnflux_kernel<T, DN_TJ_TALL, DN_NY_TALL>), grid,
+                       dim3(DN_W, DN_NY_TALL), 0, as_stream(stream), d, mode, n4, nmax, tab, q,
+                       mass, damp, del6_u, del6_v, rarea, fx, fy, nbi);
+  else
+    hipLaunchKernelGGL((delnflux_kernel<T, DN_TJ, DN_NY>), grid, dim3(DN_W, DN_NY), 0,
+                       as_stream(stream), d, mode, n4, nmax, tab, q, mass, damp, del6_u, del6_v,
+                       rarea, fx, fy, nbi);
   FV3_HIP_CHECK_LAUNCH(who);
   return FV3_OK;
 }

@@ -14,6 +14,15 @@
 #define FV3_DN_NY 4
 #endif
 
+// delnflux_kernel, tall shape for FP32 launches: interface rows per block,
+// waves per block
+#ifndef FV3_DN_TJ_TALL
+#define FV3_DN_TJ_TALL 25
+#endif
+#ifndef FV3_DN_NY_TALL
+#define FV3_DN_NY_TALL 4
+#endif
+
 // a2b_interp_kernel: corner rows per block, waves per block
 #ifndef FV3_AB_TJ
 #define FV3_AB_TJ 16

@@ -172,6 +172,41 @@ def test_delnflux_nosg_store():
     assert_close(to_np(fy2)[sy], rfy[sy], TOL, what="fy2")
 
 
+@pytest.mark.parametrize("n", [24, 48])
+@pytest.mark.parametrize("nord", [[2, 2, 2, 2, 2, 2, 2, 2], [0, 0, 2, 2, 2, 2, 2, 2]])
+def test_delnflux_fp32_tall_blocks_against_oracle(n, nord):
+    """FP32 del-n fluxes on cube tiles with all four corners at sizes whose
+    25 and 49 interface rows take the tall 25-row blocks (one and two
+    blocks per column, each holding cube corners): within FP32 rounding of
+    the FP64 oracle on the same FP32-rounded inputs.  The nord-2 flux is a
+    sixth-order difference whose cancellations amplify the FP32 rounding of
+    its inputs (2.7e-6 of the flux scale measured), so the bound is the FP32
+    transport budget, 1e-5 of the scale; an indexing slip in the block
+    layout gives errors of order one."""
+    nz = 8
+    edges = (True, True, True, True)
+    d, _, _, factory, gi = _setup(n, nz, edges, 24)
+    d32 = {k: (v.astype(np.float32).astype(np.float64) if isinstance(v, np.ndarray) else v)
+           for k, v in d.items()}
+    grid32, damp32 = grid_from_numpy(d32, dtype=torch.float32)
+    obj = DelnFluxNoSG(factory, damp32, grid32.rarea, nord[:nz])
+    dampcol = np.linspace(0.5, 1.5, nz)
+    q = to_storage(d32["q"], dtype=torch.float32)
+    fx2 = utils.zeros(gi.max_shape, dtype=np.float32)
+    fy2 = utils.zeros(gi.max_shape, dtype=np.float32)
+    obj(q, fx2, fy2, dampcol)
+    torch.cuda.synchronize()
+    rfx, rfy = ofv.delnflux_nosg(d32["q"][:, :, :nz], d32["del6_u"], d32["del6_v"],
+                                 d32["rarea"], nord[:4], dampcol, _tile(n, edges))
+    sx = (slice(3, n + 4), slice(3, n + 3), slice(0, nz))
+    sy = (slice(3, n + 3), slice(3, n + 4), slice(0, nz))
+    for got, ref, what in ((fx2, rfx, "fx2"), (fy2, rfy, "fy2")):
+        box = sx if what == "fx2" else sy
+        g = to_np(got)[box].astype(np.float64)
+        err = np.abs(g - ref[box]).max() / np.abs(ref[box]).max()
+        assert err < 1e-5, (what, err)
+
+
 @pytest.mark.parametrize("iord", [5, 6, 8])
 def test_standalone_xppm_yppm(iord):
     n, nz = 24, 6
