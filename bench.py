@@ -82,8 +82,8 @@ def _parse():
                    help="ensemble: time only the eager calls (default: also the one-graph "
                         "form, which gives value)")
     p.add_argument("--mixed", action="store_true",
-                   help="dyncore: the mixed-precision form (FP32 bulk fields); reports the error of "
-                        "every field against the FP64 form")
+                   help="dyncore / fv_dynamics: the mixed-precision form (FP32 bulk fields); "
+                        "reports its error against the FP64 form")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=10.0)
     p.add_argument("--copy-calibration", action="store_true",
@@ -397,9 +397,16 @@ def bench_dyncore(steps: int, warmup: int, roofline_kernel: str, graph: bool = F
     }
 
 
-def bench_fv_dynamics(steps: int, warmup: int, roofline_kernel: str):
+def bench_fv_dynamics(steps: int, warmup: int, roofline_kernel: str, mixed: bool = False):
     """DynamicalCore.step_dynamics on six C128 tiles x 79 levels: n_split = 6
-    acoustic substeps, 8 tracers advected (hord_tr = 8), remap with kord 10."""
+    acoustic substeps, 8 tracers advected (hord_tr = 8), remap with kord 10.
+    With ``mixed`` both forms are timed, ``value`` is the mixed form's rate
+    (acoustic substeps in the mixed form, tracers and remap in FP64), and
+    the line adds the FP64 rate, their ratio and the mixed form's errors in
+    ps, pt and delp against the FP64 form after one step from the same
+    state."""
+    import gc
+
     import torch
 
     from fv3core_amd import _lib
@@ -430,35 +437,55 @@ def bench_fv_dynamics(steps: int, warmup: int, roofline_kernel: str):
     damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
     comm = CubedSphereCommunicator(gi)
     phis = dev(init["phis"], 1)
-    core = DynamicalCore(comm, grid, factory, damp, cfg, ak, bk, phis, skip_wrapup=True)
-    state = {k: dev(v, None) for k, v in st0.items() if k != "cappa"}
-    state.update({k: dev(v, None) for k, v in cube_tracers(st0["delp"].shape).items()})
-    state["phis"] = phis
-    state["ps"] = dev(st0["pe"][..., nz].copy(), 1)
+    tracers0 = cube_tracers(st0["delp"].shape)
+
+    def fresh():
+        state = {k: dev(v, None) for k, v in st0.items() if k != "cappa"}
+        state.update({k: dev(v, None) for k, v in tracers0.items()})
+        state["phis"] = phis
+        state["ps"] = dev(st0["pe"][..., nz].copy(), 1)
+        return state
+
     bdt = float(cfg.dt_atmos)
     lib = _lib.lib()
-    for _ in range(warmup):
+
+    def run(form_mixed):
+        core = DynamicalCore(comm, grid, factory, damp, cfg, ak, bk, phis, skip_wrapup=True,
+                             mixed_precision=form_mixed)
+        state = fresh()
         core.step_dynamics(state, 0.0, False, bdt, ptop, n_split, 0)
-    torch.cuda.synchronize()
-    lib.fv3_timing_arm(roofline_kernel.encode())
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        core.step_dynamics(state, 0.0, False, bdt, ptop, n_split, 0)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    cnt, kern_ms, kern_bytes = _timing(lib, roofline_kernel)
-    lib.fv3_timing_arm(None)
-    finite = bool(torch.isfinite(state["pt"][:, 3:n + 3, 3:n + 3, :nz]).all().item())
+        torch.cuda.synchronize()
+        one_step = {k: state[k].clone() for k in ("pt", "delp", "pe")}
+        for _ in range(warmup):
+            core.step_dynamics(state, 0.0, False, bdt, ptop, n_split, 0)
+        torch.cuda.synchronize()
+        lib.fv3_timing_arm(roofline_kernel.encode())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            core.step_dynamics(state, 0.0, False, bdt, ptop, n_split, 0)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        timing = _timing(lib, roofline_kernel)
+        lib.fv3_timing_arm(None)
+        finite = bool(torch.isfinite(state["pt"][:, 3:n + 3, 3:n + 3, :nz]).all().item())
+        del core, state
+        gc.collect()
+        torch.cuda.empty_cache()
+        return wall, timing, finite, one_step
+
+    results = {form: run(form == "mixed") for form in (("fp64", "mixed") if mixed else ("fp64",))}
+    wall, (cnt, kern_ms, kern_bytes), finite, one = results["mixed" if mixed else "fp64"]
     cells = 6 * n * n * nz
     achieved = kern_bytes / max(kern_ms * 1e-3, 1e-30) / 1e9
-    return {
+    out = {
         "value": steps * cells / wall,
         "ms_per_step": wall / steps * 1e3,
         "config": {
             "workload": f"DynamicalCore.step_dynamics on 6 cube tiles x {n}x{n}x{nz}: "
             f"n_split={n_split} acoustic substeps, 8 tracers (hord_tr=8), remap kord=10 "
-            "with moist_cv, dt=225 s",
+            "with moist_cv, dt=225 s" + ("; mixed form: acoustic substeps with " + MIXED_FORM +
+                                         ", tracer transport and remap FP64" if mixed else ""),
             "tiles": 6, "n": n, "nz": nz, "n_split": n_split, "tracers": 8,
             "cells_per_step": cells, "acoustic_cells_per_step": cells * n_split,
             "state_finite_after_timed_steps": finite,
@@ -475,8 +502,40 @@ def bench_fv_dynamics(steps: int, warmup: int, roofline_kernel: str):
             "kernel_avg_us": kern_ms / max(cnt, 1) * 1e3,
             "algorithmic_bytes_per_launch": kern_bytes / max(cnt, 1),
         },
-        "dtype": "f64",
+        "dtype": "f64+f32" if mixed else "f64",
     }
+    if mixed:
+        wall64, _, finite64, one64 = results["fp64"]
+        c = (slice(None), slice(3, n + 3), slice(3, n + 3))
+
+        def rel(a, b):
+            a, b = a.double(), b.double()
+            den = torch.clamp(a.abs() + b.abs(), min=1e-6 * float(b.abs().max()) + 1e-300)
+            return float((2.0 * (a - b).abs() / den).max())
+
+        def share(a, b, tol=1e-5):
+            a, b = a.double(), b.double()
+            den = torch.clamp(a.abs() + b.abs(), min=1e-6 * float(b.abs().max()) + 1e-300)
+            return float((2.0 * (a - b).abs() / den > tol).double().mean())
+
+        pairs = {k: (one[k][c + (slice(0, nz),)], one64[k][c + (slice(0, nz),)])
+                 for k in ("pt", "delp")}
+        pairs["ps"] = (one["pe"][c + (nz,)], one64["pe"][c + (nz,)])
+        errs = {k: rel(a, b) for k, (a, b) in pairs.items()}
+        # the kord-10 remap switches its reconstruction discontinuously (one
+        # ulp on its inputs moves single levels by up to 1e-4), so the
+        # pointwise maximum is set by a few switched points: the share of
+        # points above the gate is given beside it
+        errs["share_above_1e-5"] = {k: share(a, b) for k, (a, b) in pairs.items()}
+        out.update({
+            "fp64_value": steps * cells / wall64,
+            "fp64_ms_per_step": wall64 / steps * 1e3,
+            "mixed_over_fp64": wall64 / wall,
+            "mixed_vs_fp64": errs,
+            "mixed_gate_ok": max(errs[k] for k in ("pt", "delp", "ps")) <= 1e-5,
+        })
+        out["config"]["state_finite_after_timed_steps"] = finite and finite64
+    return out
 
 
 def bench_fvtp2d(steps: int, warmup: int, dtype_name: str, copy_calibration: bool):
@@ -882,7 +941,7 @@ def main():
     elif args.workload == "fv_dynamics":
         steps = 5 if args.steps is None else args.steps
         warmup = 1 if args.warmup is None else args.warmup
-        res = bench_fv_dynamics(steps, warmup, args.roofline_kernel)
+        res = bench_fv_dynamics(steps, warmup, args.roofline_kernel, args.mixed)
         cpu = None
     elif args.workload == "ensemble":
         steps = 3 if args.steps is None else args.steps

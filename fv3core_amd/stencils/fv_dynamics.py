@@ -18,12 +18,13 @@ from __future__ import annotations
 
 import ctypes
 import types
-from typing import Mapping
+from typing import Dict, Mapping
 
 import numpy as np
 import torch
 
 from .. import _lib
+from .. import mixed as _mixed
 from ..constants import KAPPA, ZVIR
 from ..utils import gt4py_utils as utils
 from ..utils.grid import DampingCoefficients, GridData
@@ -85,8 +86,16 @@ class DynamicalCore:
 
     def __init__(self, comm, grid_data: GridData, stencil_factory: StencilFactory,
                  damping_coefficients: DampingCoefficients, config, ak, bk, phis,
-                 use_graph: bool = False, skip_wrapup: bool = False):
-        """``use_graph`` (no counterpart in the reference): capture a whole
+                 use_graph: bool = False, skip_wrapup: bool = False,
+                 mixed_precision: bool = False):
+        """``mixed_precision`` (no counterpart in the reference, which is FP64
+        only): run the acoustic substeps in the mixed form (FP32 bulk
+        storages, FP64 solvers, pressures and heights); the step's state stays
+        FP64: before each k_split iteration's acoustic call its bulk fields
+        are rounded into FP32 copies the component owns, and after it widened
+        back, so the tracer transport and the remap run in FP64 on the
+        acoustic step's results.
+        ``use_graph`` (no counterpart in the reference): capture a whole
         step (every k_split iteration of the acoustic substeps, tracer
         transport and remap, and the wrap-up) in one HIP graph and replay it
         on later steps with the same storages and scalars; the first step for
@@ -96,6 +105,8 @@ class DynamicalCore:
         gi = stencil_factory.grid_indexing
         self._skip_wrapup = bool(skip_wrapup)
         self._use_graph = bool(use_graph)
+        self._mixed = bool(mixed_precision)
+        self._bulk32: Dict[str, torch.Tensor] = {}
         self._graphs = {}
         self._seen_keys = set()
         if not config.moist_phys:
@@ -127,7 +138,7 @@ class DynamicalCore:
         self.acoustic_dynamics = AcousticDynamics(
             comm, stencil_factory, grid_data, damping_coefficients, config.grid_type, False,
             False, config.acoustic_dynamics, self._ak_host, self._bk_host, self._pfull,
-            self._phis)
+            self._phis, mixed_precision=self._mixed)
         self._hyperdiffusion = HyperdiffusionDamping(stencil_factory, damping_coefficients,
                                                      grid_data.rarea, config.nf_omega)
         area_64 = grid_data.area_64 if grid_data.has("area_64") else grid_data.area
@@ -142,6 +153,27 @@ class DynamicalCore:
         self._temporaries = None
 
     # ------------------------------------------------------------ helpers
+    def _acoustic(self, state):
+        """AcousticDynamics on the FP64 state, or on its mixed-precision form."""
+        if not self._mixed:
+            self.acoustic_dynamics(state)
+            return
+        ms = types.SimpleNamespace()
+        for k, v in vars(state).items():
+            if k in _mixed.BULK_FIELDS and isinstance(v, torch.Tensor):
+                t = self._bulk32.get(k)
+                if t is None or t.shape != v.shape or t.stride() != v.stride():
+                    t = torch.empty_like(v, dtype=torch.float32)
+                    self._bulk32[k] = t
+                _mixed.cast_into(v, t)
+                setattr(ms, k, t)
+            else:
+                setattr(ms, k, v)
+        self.acoustic_dynamics(ms)
+        for k, t in self._bulk32.items():
+            if isinstance(state.__dict__.get(k), torch.Tensor):
+                _mixed.cast_into(t, state.__dict__[k])
+
     def _make_temporaries(self, like: torch.Tensor):
         """fvdyn_temporaries (fv_dynamics.py:172-190)."""
         shape3 = tuple(like.shape)
@@ -252,7 +284,7 @@ class DynamicalCore:
             last_step = n_map == state.k_split - 1
             self._copy(state.delp, state.dp1)
             with timer.clock("DynCore"):
-                self.acoustic_dynamics(state)
+                self._acoustic(state)
             if config.z_tracer:
                 with timer.clock("TracerAdvection"):
                     self.tracer_advection(tracers, state.dp1, state.mfxd, state.mfyd, state.cxd,

@@ -303,6 +303,87 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup, sat_adj):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_gpu_mixed_full_step_against_fp64_oracle(use_graph):
+    """DynamicalCore(mixed_precision=True): the acoustic substeps in the mixed
+    form, tracer transport and remap in FP64 (k_split 2, kord 9, 8 tracers),
+    against the FP64 oracle's step: ps, pt and delp within the 1e-5 gate,
+    the other fields' errors stated (fields that pass through zero against
+    their own scale).  With use_graph the second and third steps replay one
+    HIP graph and must equal the eager steps bit for bit."""
+    from fv3core_amd.halo import CubedSphereCommunicator
+    from fv3core_amd.stencils.fv_dynamics import DynamicalCore
+    from fv3core_amd.utils import gt4py_utils as utils
+    from fv3core_amd.utils.grid import DampingCoefficients, GridData
+    from fv3core_amd.utils.stencil import GridIndexing, StencilConfig, StencilFactory
+
+    init = moist_cube_state(N, NZ, seed=4)
+    st0, tr0, m = init["state"], init["tracers"], init["metric"]
+    ak, bk, ptop = init["ak"], init["bk"], init["ptop"]
+    cfg = _config(k_split=2, kord=9)
+    gi = GridIndexing((N, N, NZ), 3, True, True, True, True)
+    factory = StencilFactory(StencilConfig(), gi)
+
+    def dev(a, bd=None):
+        t = utils.empty(a.shape, batch_dims=bd)
+        t.copy_(torch.as_tensor(a))
+        return t
+
+    grid = GridData({k: dev(v, 1) for k, v in m.items()
+                     if isinstance(v, np.ndarray) and v.ndim == 3})
+    damp = DampingCoefficients(grid.del6_u, grid.del6_v, m["da_min"], m["da_min_c"])
+    phis = dev(init["phis"], 1)
+
+    def fresh():
+        state = {k: dev(v) for k, v in st0.items() if k not in ("cappa",)}
+        state.update({k: dev(v) for k, v in tr0.items()})
+        state["phis"] = phis
+        state["ps"] = dev(st0["pe"][..., NZ].copy(), 1)
+        return state
+
+    bdt = float(cfg.dt_atmos)
+    core = DynamicalCore(CubedSphereCommunicator(gi), grid, factory, damp, cfg, ak, bk, phis,
+                         skip_wrapup=True, mixed_precision=True)
+    state = fresh()
+    core.step_dynamics(state, 0.0, False, bdt, ptop, cfg.n_split, 0)
+    torch.cuda.synchronize()
+    S = _oracle_step(init, cfg, bdt, ptop, None)
+    c = (slice(None), slice(3, N + 3), slice(3, N + 3), slice(0, NZ))
+    errs = {}
+    for name in ("delp", "pt", "delz", "q_con", "pkz") + TRACERS[:8]:
+        a, b = state[name].cpu().numpy()[c], S[name][c]
+        floor = 1e-6 * np.abs(b).max()
+        errs[name] = float((2 * np.abs(a - b) / np.maximum(np.abs(a) + np.abs(b), floor)).max())
+    for name, box in (("w", c), ("omga", c),
+                      ("u", (slice(None), slice(3, N + 3), slice(3, N + 4), slice(0, NZ))),
+                      ("v", (slice(None), slice(3, N + 4), slice(3, N + 3), slice(0, NZ)))):
+        a, b = state[name].cpu().numpy()[box], S[name][box]
+        errs[name + "/scale"] = float(np.abs(a - b).max() / np.abs(b).max())
+    ps = state["pe"].cpu().numpy()[:, 3:N + 3, 3:N + 3, NZ]
+    ps_ref = S["pe"][:, 3:N + 3, 3:N + 3, NZ]
+    errs["ps"] = float((np.abs(ps - ps_ref) / np.abs(ps_ref)).max())
+    print("mixed full step vs FP64 oracle: " + ", ".join(f"{k} {v:.1e}" for k, v in errs.items()))
+    for k in ("ps", "pt", "delp"):
+        assert errs[k] <= 1e-5, f"{k}: {errs[k]:.2e} above the 1e-5 gate"
+    assert all(np.isfinite(v) for v in errs.values())
+    assert errs["delp"] > 0.0  # the acoustic bulk really is FP32
+    if use_graph:
+        eager = DynamicalCore(CubedSphereCommunicator(gi), grid, factory, damp, cfg, ak, bk,
+                              phis, skip_wrapup=True, mixed_precision=True)
+        graph = DynamicalCore(CubedSphereCommunicator(gi), grid, factory, damp, cfg, ak, bk,
+                              phis, skip_wrapup=True, mixed_precision=True, use_graph=True)
+        se, sg = fresh(), fresh()
+        for step in range(3):
+            eager.step_dynamics(se, 0.0, False, bdt, ptop, cfg.n_split, 0)
+            graph.step_dynamics(sg, 0.0, False, bdt, ptop, cfg.n_split, 0)
+            torch.cuda.synchronize()
+            for k in se:
+                torch.testing.assert_close(sg[k], se[k], rtol=0, atol=0, equal_nan=True,
+                                           msg=f"step {step}: {k}")
+        assert graph._graphs
+
+
+@pytest.mark.gpu
 def test_gpu_dynamical_core_graph_replay_matches_eager():
     """DynamicalCore(use_graph=True): the whole step (acoustic substeps,
     batched tracer transport, remap, omega) captured in one HIP graph on the
