@@ -558,55 +558,69 @@ int nh_p_grad(const fv3_domain* dom, const fv3_metrics* g, S* u, S* v, double* p
               FV3_E_ARG, "%s: work_slab too small", who);
   hipStream_t s = as_stream(stream);
   const Met<double> m = to_met<double>(g);
-  double *wk = work, *wk1 = work + work_slab, *qx = work + 2 * work_slab;
-  double* edges = work + 4 * work_slab;
-  // a2b_k1 on pp and pk3 (levels 1..nz), a2b_kbuffer on gz (0..nz), replace
-  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pp, wk1, qx, edges, 1, pp, who);
+  // slabs: B-grid pp, delp's B-grid values (wk1), the a2b scratch (corner
+  // weights, edges), B-grid pk3 and gz
+  double *ppb = work, *wk1 = work + work_slab, *qx = work + 2 * work_slab;
+  double *pk3b = work + 3 * work_slab, *edges = work + 4 * work_slab;
+  double* gzb = work + 5 * work_slab;
+  // a2b_k1 on pp and pk3 (levels 1..nz), a2b_kbuffer on gz (0..nz), into
+  // the slabs; the u/v update below writes them back into pp, pk3 and gz
+  // (the reference's replace), so no separate copies are launched
+  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pp, ppb, qx, edges, 0, nullptr, who);
   if (rc) return rc;
-  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pk3, wk1, qx, edges, 1, pk3, who);
+  rc = launch_a2b_ord4<double>(s, dom, m, 1, nz, pk3, pk3b, qx, edges, 0, nullptr, who);
   if (rc) return rc;
-  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz + 1, gz, wk1, qx, edges, 1, gz, who);
+  rc = launch_a2b_ord4<double>(s, dom, m, 0, nz + 1, gz, gzb, qx, edges, 0, nullptr, who);
   if (rc) return rc;
   rc = launch_a2b_ord4<double>(s, dom, m, 0, nz, delp, wk1, qx, edges, 0, nullptr, who);
   if (rc) return rc;
   Dom d = to_dom(dom);
-  // set_k0_and_calc_wk on [isc, iec+1]^2
+  // set_k0_and_calc_wk, calc_u and calc_v on [isc, iec+1]^2 in one pass:
+  // level 0 of pp and pk3 is 0 and ptk, wk = pk3(k+1) - pk3(k) is formed
+  // where it is read (the same subtraction), and every point writes its
+  // B-grid pp, pk3 and gz back (the top thread also level nz)
   const Box bc{d.isc, d.iec + 1, d.jsc, d.jec + 1};
-  launch_box(s, d, bc, 0, nz, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
-    if (k == 0) {
-      pp[o] = 0.0;
-      pk3[o] = ptk;
-    }
-    wk[o] = pk3[o + d.sk] - pk3[o];
-  });
-  FV3_HIP_CHECK_LAUNCH(who);
-  // calc_u on [isc, iec] x [jsc, jec+1] and calc_v on [isc, iec+1] x [jsc, jec]
   launch_box(s, d, bc, 0, nz, [=] __device__(int i, int j, int k, int t) {
     const int64_t o = at3(d, i, j, k, t);
     const int64_t o2 = at2(d, i, j, t);
     const int64_t sk = d.sk;
+    auto pk3_at = [&](int64_t p, int kk) { return kk == 0 ? ptk : pk3b[p]; };
+    auto pp_at = [&](int64_t p, int kk) { return kk == 0 ? 0.0 : ppb[p]; };
+    const double pk3_o = pk3_at(o, k), pk3_ok = pk3b[o + sk];
+    const double wk_o = pk3_ok - pk3_o;
     if (i <= d.iec) {
       const int64_t q = o + 1;
-      const double du = dt / (wk[o] + wk[q]) *
-                        ((gz[o + sk] - gz[q]) * (pk3[q + sk] - pk3[o]) +
-                         (gz[o] - gz[q + sk]) * (pk3[o + sk] - pk3[q]));
+      const double pk3_q = pk3_at(q, k), pk3_qk = pk3b[q + sk];
+      const double wk_q = pk3_qk - pk3_q;
+      const double du = dt / (wk_o + wk_q) *
+                        ((gzb[o + sk] - gzb[q]) * (pk3_qk - pk3_o) +
+                         (gzb[o] - gzb[q + sk]) * (pk3_ok - pk3_q));
       u[o] = (S)(((double)u[o] + du +
               dt / (wk1[o] + wk1[q]) *
-                  ((gz[o + sk] - gz[q]) * (pp[q + sk] - pp[o]) +
-                   (gz[o] - gz[q + sk]) * (pp[o + sk] - pp[q]))) *
+                  ((gzb[o + sk] - gzb[q]) * (ppb[q + sk] - pp_at(o, k)) +
+                   (gzb[o] - gzb[q + sk]) * (ppb[o + sk] - pp_at(q, k)))) *
              m.rdx[o2]);
     }
     if (j <= d.jec) {
       const int64_t q = o + d.sj;
-      const double dv = dt / (wk[o] + wk[q]) *
-                        ((gz[o + sk] - gz[q]) * (pk3[q + sk] - pk3[o]) +
-                         (gz[o] - gz[q + sk]) * (pk3[o + sk] - pk3[q]));
+      const double pk3_q = pk3_at(q, k), pk3_qk = pk3b[q + sk];
+      const double wk_q = pk3_qk - pk3_q;
+      const double dv = dt / (wk_o + wk_q) *
+                        ((gzb[o + sk] - gzb[q]) * (pk3_qk - pk3_o) +
+                         (gzb[o] - gzb[q + sk]) * (pk3_ok - pk3_q));
       v[o] = (S)(((double)v[o] + dv +
               dt / (wk1[o] + wk1[q]) *
-                  ((gz[o + sk] - gz[q]) * (pp[q + sk] - pp[o]) +
-                   (gz[o] - gz[q + sk]) * (pp[o + sk] - pp[q]))) *
+                  ((gzb[o + sk] - gzb[q]) * (ppb[q + sk] - pp_at(o, k)) +
+                   (gzb[o] - gzb[q + sk]) * (ppb[o + sk] - pp_at(q, k)))) *
              m.rdy[o2]);
+    }
+    pp[o] = pp_at(o, k);
+    pk3[o] = pk3_o;
+    gz[o] = gzb[o];
+    if (k == nz - 1) {
+      pp[o + sk] = ppb[o + sk];
+      pk3[o + sk] = pk3_ok;
+      gz[o + sk] = gzb[o + sk];
     }
   });
   FV3_HIP_CHECK_LAUNCH(who);

@@ -26,7 +26,7 @@ class NonHydrostaticPressureGradient:
         self._gi = stencil_factory.grid_indexing
         self._grid = grid_data
         self.nk = self._gi.domain[2]
-        self._work = Workspace(5)
+        self._work = Workspace(6)
 
     def __call__(self, u, v, pp, gz, pk3, delp, dt: float, ptop: float, akap: float):
         """u, v updated; pp, gz, pk3 replaced on the B grid (as the reference does)."""

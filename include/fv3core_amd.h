@@ -516,7 +516,7 @@ int32_t fv3_update_dz_d_mixed(const fv3_domain* d, const fv3_metrics* g, int32_t
  * NonHydrostaticPressureGradient.__call__ (fv3core/stencils/nh_p_grad.py:147-206)
  * on d->nk layers: pp, pk3 (levels 1..nk) and gz (0..nk) replaced by their
  * B-grid values, then u, v updated (and multiplied by rdx, rdy).
- * ptk = ptop ** akap.  work: 5 slabs of nk + 1 levels.
+ * ptk = ptop ** akap.  work: 6 slabs of nk + 1 levels.
  */
 int32_t fv3_nh_p_grad_f64(const fv3_domain* d, const fv3_metrics* g, double* u, double* v,
                           double* pp, double* gz, double* pk3, const double* delp, double dt,
