@@ -715,7 +715,7 @@ void append_rotation_fill(FillTable* tab, const fv3_domain* dom, int kind, int f
 template <typename T>
 int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv3_dsw_params* p,
                  const DswCols<T>& cols, T dt, const T* wk, T* divg, T* vc, T* uc, T* delpc,
-                 T* vort, T* ke, T* a2b_qx, T* a2b_edges, const char* who) {
+                 T* vort, T* ke, T* a2b_qx, T* a2b_edges, T* dg_last, const char* who) {
   const int k0 = p->div_k0;
   fv3_domain hd = *dom;
   hd.nk = dom->nk - k0;
@@ -730,6 +730,7 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
   T* keh = ke + ko;
   const T* wkh = wk + ko;
   const T* d2 = cols.d2_divg + k0;
+  T* dgl = dg_last + ko;
   const int nord = p->div_nord;
   const bool corner = (has_edge(dom->i_start) || has_edge(dom->i_end)) &&
                       (has_edge(dom->j_start) || has_edge(dom->j_end));
@@ -740,18 +741,23 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
   // the factor is 1, which leaves every result bit for bit as it was.
   const bool rescale = sizeof(T) < sizeof(double);
   const T pass_scale = rescale ? (T)p->da_min_c : T(1);
-  // copy_computeplus: delpc = divg_d on the corners (before the passes change divg_d)
+  // copy_computeplus (delpc = divg_d on the corners, before the passes
+  // change divg_d) is done by the first pass's x differences, whose box
+  // holds the corners; the corner fills before them write only halo points
   const Box comp{d.isc, d.iec + 1, d.jsc, d.jec + 1};
-  launch_box(s, d, comp, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
-    const int64_t o = at3(d, i, j, k, t);
-    dpc[o] = dg[o];
-  });
-  FV3_HIP_CHECK_LAUNCH(who);
+  if (nord == 0) {
+    launch_box(s, d, comp, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      dpc[o] = dg[o];
+    });
+    FV3_HIP_CHECK_LAUNCH(who);
+  }
   FillFields<T> ff{};
   ff.f[0] = dg;
   ff.f[1] = vch;
   ff.f[2] = uch;
-  for (int n = 0; n < nord; ++n) {
+  // the last pass (nt = 0, no corner fills) is one launch: below
+  for (int n = 0; n + 1 < nord; ++n) {
     const int nt = nord - (n + 1);
     const bool fillc = (n + 1 != nord) && corner;
     if (fillc) {
@@ -762,9 +768,11 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
       FV3_HIP_CHECK_LAUNCH(who);
     }
     const Box bv{d.isc - nt - 1, d.iec + nt + 1, d.jsc - nt, d.jec + nt + 1};
+    const bool first = n == 0;
     launch_box(s, d, bv, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
       const int64_t o = at3(d, i, j, k, t);
       vch[o] = (dg[o + 1] - dg[o]) * m.divg_u[at2(d, i, j, t)];
+      if (first && i >= d.isc && i <= d.iec + 1 && j >= d.jsc && j <= d.jec + 1) dpc[o] = dg[o];
     });
     FV3_HIP_CHECK_LAUNCH(who);
     if (fillc) {
@@ -804,6 +812,44 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
     });
     FV3_HIP_CHECK_LAUNCH(who);
   }
+  // the last pass: vc on [isc-1, iec+1] x [jsc, jec+1], uc on
+  // [isc, iec+1] x [jsc-1, jec+1] and the new divg_d on the corners, the
+  // differences it reads formed again from divg_d where they lie (the same
+  // expressions); the new divg_d goes to dg_last (its neighbours still read
+  // the old one) and the final step below writes it back
+  if (nord > 0) {
+    const bool copy = nord == 1;
+    const Box bl{d.isc - 1, d.iec + 1, d.jsc - 1, d.jec + 1};
+    launch_box(s, d, bl, 0, d.nk, [=] __device__(int i, int j, int k, int t) {
+      const int64_t o = at3(d, i, j, k, t);
+      const int64_t sj = d.sj;
+      auto vcx = [&](int64_t q, int ii, int jj) {
+        return (dg[q + 1] - dg[q]) * m.divg_u[at2(d, ii, jj, t)];
+      };
+      auto ucy = [&](int64_t q, int ii, int jj) {
+        return (dg[q + sj] - dg[q]) * m.divg_v[at2(d, ii, jj, t)];
+      };
+      const bool in_v = j >= d.jsc;   // i in [isc-1, iec+1], j in [jsc, jec+1]
+      const bool in_u = i >= d.isc;   // i in [isc, iec+1], j in [jsc-1, jec+1]
+      if (in_v) vch[o] = vcx(o, i, j);
+      if (in_u) uch[o] = ucy(o, i, j);
+      if (!(in_u && in_v)) return;
+      if (copy) dpc[o] = dg[o];
+      const T v_w = vcx(o - 1, i - 1, j), v_e = vcx(o, i, j);
+      const T u_s = ucy(o - sj, i, j - 1), u_n = ucy(o, i, j);
+      T val;
+      const bool ci = i == d.is || i == d.ie + 1;
+      if (ci && j == d.js) {
+        val = v_w - v_e - u_n;
+      } else if (ci && j == d.je + 1) {
+        val = u_s + v_w - v_e;
+      } else {
+        val = u_s - u_n + v_w - v_e;
+      }
+      dgl[o] = val * m.rarea_c[at2(d, i, j, t)] * pass_scale;
+    });
+    FV3_HIP_CHECK_LAUNCH(who);
+  }
   // _vorticity_calc
   const T dddmp = (T)p->dddmp;
   if (!(dddmp < T(1e-5))) {
@@ -825,7 +871,14 @@ int run_div_high(hipStream_t s, const fv3_domain* dom, const Met<T>& m, const fv
       vv = adt * sqrt(dp * dp + a * a);
     }
     const T damp = damp_tmp(vv, da_min_c, d2[k], dddmp);
-    const T nv = damp * dp + dd8 * dg[o];
+    T g;
+    if (nord > 0) {
+      g = dgl[o];
+      dg[o] = g;
+    } else {
+      g = dg[o];
+    }
+    const T nv = damp * dp + dd8 * g;
     vo[o] = nv;
     keh[o] = keh[o] + nv;
   });
@@ -923,7 +976,7 @@ int run_d_sw(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_params* 
   T *ucc = W[0], *vcc = W[1], *fx = W[2], *fy = W[3], *gx = W[4], *gy = W[5];
   T *fx2 = W[6], *fy2 = W[7], *dw = W[8], *heat_s = W[9], *wk = W[10], *vort = W[11];
   T *ke = W[12], *vxd = W[13], *vyd = W[14], *ut = W[15], *vt = W[16], *qy = W[17];
-  T *a2x = W[18], *a2e = W[20], *vort2 = W[21];  // W[19] is free
+  T *a2x = W[18], *a2e = W[20], *vort2 = W[21];  // W[19]: the last divergence pass
   const T* area = m.area;
   const T* dxa = m.dxa;
   const T* dya = m.dya;
@@ -990,7 +1043,7 @@ int run_d_sw(const fv3_domain* dom, const fv3_metrics* g, const fv3_dsw_params* 
     FV3_HIP_CHECK_LAUNCH(who);
   }
   rc = run_div_high<T>(s, dom, m, p, cols, dt, wk, st.divgd, st.vc, st.uc, st.delpc, vort, ke,
-                       a2x, a2e, who);
+                       a2x, a2e, W[19], who);
   if (rc) return rc;
   launch_vort_diff<T>(s, d, m, vort, wk, vxd, vyd, vort2, cols.d_con_col);
   FV3_HIP_CHECK_LAUNCH(who);
