@@ -129,12 +129,12 @@ def test_gpu_acoustic_dynamics_matches_oracle(nz, n_split):
 
     def dev4(a):
         t = utils.empty(a.shape)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     def dev3(a):
         t = utils.empty(a.shape, batch_dims=1)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     grid = GridData({k: dev3(v) for k, v in m.items()
@@ -210,7 +210,7 @@ def test_gpu_acoustic_dynamics_graph_replay_matches_eager():
 
     def dev(a, bd):
         t = utils.empty(a.shape, batch_dims=bd)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     grid = GridData({k: dev(v, 1) for k, v in m.items()

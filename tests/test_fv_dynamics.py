@@ -222,12 +222,12 @@ def test_gpu_dynamical_core_step_matches_oracle(k_split, kord, wrapup, sat_adj):
 
     def dev4(a):
         t = utils.empty(a.shape)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     def dev3(a):
         t = utils.empty(a.shape, batch_dims=1)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     grid = GridData({k: dev3(v) for k, v in m.items()
@@ -326,7 +326,7 @@ def test_gpu_mixed_full_step_against_fp64_oracle(use_graph):
 
     def dev(a, bd=None):
         t = utils.empty(a.shape, batch_dims=bd)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     grid = GridData({k: dev(v, 1) for k, v in m.items()

@@ -92,7 +92,7 @@ def _run_mixed_and_oracle(nz, n_split, seed=2):
 
     def dev(a, bd):
         t = utils.empty(a.shape, batch_dims=bd)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     grid = GridData({k: dev(v, 1) for k, v in m.items()
@@ -194,7 +194,7 @@ def test_gpu_mixed_step_needs_a_mixed_state():
 
     def dev(a, bd):
         t = utils.empty(a.shape, batch_dims=bd)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     grid = GridData({k: dev(v, 1) for k, v in m.items()

@@ -161,7 +161,7 @@ def test_oracle_hand_worked_cells():
 def _device(a):
     from fv3core_amd.utils import gt4py_utils as utils
     t = utils.empty(a.shape, batch_dims=1 if a.ndim == 3 else None)
-    t.copy_(torch.as_tensor(a))
+    t.copy_(torch.as_tensor(np.array(a)))
     return t
 
 

@@ -62,12 +62,12 @@ def test_gpu_tracer_advection_matches_oracle():
 
     def dev4(a):
         t = utils.empty(a.shape)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     def dev3(a):
         t = utils.empty(a.shape, batch_dims=1)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     grid = GridData({k: dev3(v) for k, v in metric.items()})
@@ -106,7 +106,7 @@ def test_gpu_batched_tracer_transport_equals_one_by_one(ntr):
 
     def dev(a, bd=None):
         t = utils.empty(a.shape, batch_dims=bd)
-        t.copy_(torch.as_tensor(a))
+        t.copy_(torch.as_tensor(np.array(a)))
         return t
 
     grid = GridData({k: dev(v, 1) for k, v in metric.items()})
