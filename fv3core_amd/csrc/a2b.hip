@@ -191,20 +191,52 @@ __global__ void __launch_bounds__(AB_W * AB_NY)
     sq[r][li] = (col_ok && j >= 0 && j < d.nj) ? qin[off3 + (int64_t)j * d.sj + i] : T(0);
   }
   __syncthreads();
-  // x means where launch 2 formed them: i in [isc, iec+1], j in [jsc-2, jec+2]
+  // x means where launch 2 formed them: i in [isc, iec+1], j in [jsc-2, jec+2].
+  // The lanes of one wave span the whole row, so the one-sided forms of the
+  // tile-edge interfaces diverge within it.  They share one instruction
+  // stream here: the outer form at is equals the one at ie+1 (the same two
+  // halves, added in the other order), and the inner form at is+1 or ie is
+  // the outer form of its neighbour combined with interior means, mirrored
+  // about the interface for the east side.  Each value keeps the operations
+  // of volume_mean, so the same bits; a wave runs 6 divisions a row instead
+  // of one set per form present in it.
   if (li >= 3 && li < AB_W - 3 && i >= d.isc && i <= d.iec + 1) {
+    // form: 0 interior, 1 outer, 2 inner west, 3 inner east, in volume_mean's
+    // order of precedence; sh: the outer form's point, relative to i
+    int form = 0, sh = 0;
+    if (i == d.ie) {
+      form = 3;
+      sh = 1;
+    } else if (i == d.ie + 1) {
+      form = 1;
+    } else if (i == d.is + 1) {
+      form = 2;
+      sh = -1;
+    } else if (i == d.is) {
+      form = 1;
+    }
+    const bool west = form == 2;
     for (int r = ty; r < AB_H - 1; r += AB_NY) {
       const int j = jb + r;
-      if (j >= d.jsc - 2 && j <= d.jec + 2) {
-        const int64_t o2 = off2 + (int64_t)j * d.sj2 + i;
-        T q7[7], dx7[7];
-#pragma unroll
-        for (int n = 0; n < 7; ++n) {
-          q7[n] = sq[r][li + n - 3];
-          dx7[n] = m.dxa[o2 + n - 3];
+      if (j < d.jsc - 2 || j > d.jec + 2) continue;
+      const T* q = &sq[r][li];
+      T v = T(kB2) * (q[-2] + q[1]) + T(kB1) * (q[-1] + q[0]);
+      if (form != 0) {
+        const T* dx = m.dxa + off2 + (int64_t)j * d.sj2 + i;
+        const T outer = qx_edge_west(q + sh, dx + sh);
+        if (form == 1) {
+          v = outer;
+        } else {
+          // qx_edge_west2 at is+1 (its left = outer) or qx_edge_east2 at ie
+          // (its right = outer), with q[k] -> q[-1-k] on the east side
+          auto qm = [&](int k) { return west ? q[k] : q[-1 - k]; };
+          const T gn = west ? dx[0] : dx[-1], gd = west ? dx[-1] : dx[0];
+          const T g = gn / gd;
+          const T inner = T(kB2) * (qm(-1) + qm(2)) + T(kB1) * (qm(0) + qm(1));
+          v = (T(3) * (g * qm(-1) + qm(0)) - (g * outer + inner)) / (T(2) + T(2) * g);
         }
-        sx[r][li] = volume_mean(q7, dx7, i, d.is, d.ie);
       }
+      sx[r][li] = v;
     }
   }
   // y means where launch 2 formed them: i in [isc-2, iec+2], j in [jsc, jec+1]
