@@ -244,14 +244,20 @@ __global__ void __launch_bounds__(AB_W * AB_NY)
     for (int r = ty; r < AB_TJ; r += AB_NY) {
       const int j = j0 + r;
       if (j >= d.jsc && j <= d.jec + 1) {
-        const int64_t o2 = off2 + (int64_t)j * d.sj2 + i;
-        T q7[7], dy7[7];
+        // the row is uniform across the wave: only the rows on the tile-edge
+        // interfaces read the cell lengths and take the one-sided forms
+        T v = T(kB2) * (sq[r + 1][li] + sq[r + 4][li]) + T(kB1) * (sq[r + 2][li] + sq[r + 3][li]);
+        if (j == d.js || j == d.js + 1 || j == d.je || j == d.je + 1) {
+          const int64_t o2 = off2 + (int64_t)j * d.sj2 + i;
+          T q7[7], dy7[7];
 #pragma unroll
-        for (int n = 0; n < 7; ++n) {
-          q7[n] = sq[r + n][li];
-          dy7[n] = m.dya[o2 + (int64_t)(n - 3) * d.sj2];
+          for (int n = 0; n < 7; ++n) {
+            q7[n] = sq[r + n][li];
+            dy7[n] = m.dya[o2 + (int64_t)(n - 3) * d.sj2];
+          }
+          v = volume_mean(q7, dy7, j, d.js, d.je);
         }
-        sy[r][li] = volume_mean(q7, dy7, j, d.js, d.je);
+        sy[r][li] = v;
       }
     }
   }
