@@ -82,11 +82,30 @@ __constant__ int kCornerTaps[4][3][4] = {
 // tile blockIdx.x: extrap_corner is qa + x1 / (x2 - x1) (qa - qb) with
 // great-circle distances that depend on the grid only, so they are formed
 // once per call here instead of once per level in the perimeter launch.
+//
+// The same launch forms, per tile and row j in [jsc-2, jec+2], the cell-length
+// ratios of the one-sided x volume means (qx_edge_west's g_in = dxa(p+1) /
+// dxa(p) and g_ou = dxa(p-2) / dxa(p-1) at p = is, then at p = ie+1), which
+// depend on the grid only; the means read them instead of dividing again on
+// every level (the same divisions, so the same bits).
 template <typename T>
-__global__ void __launch_bounds__(64) corner_weights_kernel(Met<T> m, Dom d, T* w) {
+__global__ void __launch_bounds__(64) corner_weights_kernel(Met<T> m, Dom d, T* w, T* xr,
+                                                            int nrow) {
   const int e = threadIdx.x;
-  if (e >= 12) return;
   const int t = blockIdx.x;
+  for (int r = e; r < nrow; r += 64) {
+    const T* dx = m.dxa + (int64_t)t * d.st2 + (int64_t)(d.jsc - 2 + r) * d.sj2;
+    T* out = xr + ((int64_t)t * nrow + r) * 4;
+    if (d.is >= 2 && d.is + 1 < d.ni) {
+      out[0] = dx[d.is + 1] / dx[d.is];
+      out[1] = dx[d.is - 2] / dx[d.is - 1];
+    }
+    if (d.ie >= 1 && d.ie + 2 < d.ni) {
+      out[2] = dx[d.ie + 2] / dx[d.ie + 1];
+      out[3] = dx[d.ie - 1] / dx[d.ie];
+    }
+  }
+  if (e >= 12) return;
   const int corner = e / 3, tap = e % 3;
   const int ic = (corner == 1 || corner == 2) ? d.iec + 1 : d.isc;
   const int jc = (corner >= 2) ? d.jec + 1 : d.jsc;
@@ -168,12 +187,14 @@ struct A2BArgs {
   int ci0, cj0, nci, ncj;  // corner box (interpolation off the edge lines)
   int nbi;                 // blocks along i
   int k0;
+  int nrow;                // rows of the edge ratios (jsc-2 .. jec+2)
 };
 
 template <typename T>
 __global__ void __launch_bounds__(AB_W * AB_NY)
     a2b_interp_kernel(Dom d, Met<T> m, A2BArgs a, const T* __restrict__ qin,
-                      const T* __restrict__ edges, T* __restrict__ qout) {
+                      const T* __restrict__ edges, const T* __restrict__ xr,
+                      T* __restrict__ qout) {
   __shared__ T sq[AB_H][AB_W];
   __shared__ T sx[AB_H - 1][AB_W];  // x means on rows j0-3 .. j0+TJ+1
   __shared__ T sy[AB_TJ][AB_W];     // y means on rows j0 .. j0+TJ-1
@@ -222,16 +243,22 @@ __global__ void __launch_bounds__(AB_W * AB_NY)
       const T* q = &sq[r][li];
       T v = T(kB2) * (q[-2] + q[1]) + T(kB1) * (q[-1] + q[0]);
       if (form != 0) {
-        const T* dx = m.dxa + off2 + (int64_t)j * d.sj2 + i;
-        const T outer = qx_edge_west(q + sh, dx + sh);
+        // the outer form at is or ie+1 (qx_edge_west there) from its ratios
+        const T* gr = xr + ((int64_t)t * a.nrow + (j - (d.jsc - 2))) * 4;
+        const bool wside = i <= d.is + 1;
+        const T g_in = wside ? gr[0] : gr[2], g_ou = wside ? gr[1] : gr[3];
+        const T* qb = q + sh;
+        const T outer = T(0.5) * (((T(2) + g_in) * qb[0] - qb[1]) / (T(1) + g_in) +
+                                  ((T(2) + g_ou) * qb[-1] - qb[-2]) / (T(1) + g_ou));
         if (form == 1) {
           v = outer;
         } else {
           // qx_edge_west2 at is+1 (its left = outer) or qx_edge_east2 at ie
-          // (its right = outer), with q[k] -> q[-1-k] on the east side
+          // (its right = outer), with q[k] -> q[-1-k] on the east side; its
+          // ratio dxa(is+1) / dxa(is) or dxa(ie-1) / dxa(ie) is the outer
+          // form's g_in or g_ou
           auto qm = [&](int k) { return west ? q[k] : q[-1 - k]; };
-          const T gn = west ? dx[0] : dx[-1], gd = west ? dx[-1] : dx[0];
-          const T g = gn / gd;
+          const T g = west ? g_in : g_ou;
           const T inner = T(kB2) * (qm(-1) + qm(2)) + T(kB1) * (qm(0) + qm(1));
           v = (T(3) * (g * qm(-1) + qm(0)) - (g * outer + inner)) / (T(2) + T(2) * g);
         }
@@ -305,10 +332,13 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
   const int js2 = so ? d.jsc + 1 : d.jsc, je1 = no ? d.jec : d.jec + 1;
   const int is2 = w ? d.isc + 1 : d.isc, ie1 = e ? d.iec : d.iec + 1;
   // 0: corner extrapolation weights, 12 per tile, into the head of the qx
-  // slab (launch 2 writes qx only after launch 1 has read them)
+  // slab, and the x volume means' edge ratios after them (nothing else
+  // writes that slab during the call)
   T* const cw = qx;
+  T* const xr = qx + 12 * (int64_t)d.ntiles;
+  const int nrow = d.jec - d.jsc + 5;
   hipLaunchKernelGGL((corner_weights_kernel<T>), dim3((unsigned)d.ntiles), dim3(64), 0, s, m, d,
-                     cw);
+                     cw, xr, nrow);
   FV3_HIP_CHECK_LAUNCH(who);
   // 1: corners and edge lines: only the perimeter of [isc, iec+1]^2 can hold them
   const Box bx{d.isc, d.iec + 1, d.jsc, d.jec + 1};
@@ -361,6 +391,7 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
     a.nci = (e ? d.iec : d.iec + 1) - a.ci0 + 1;
     a.ncj = (no ? d.jec : d.jec + 1) - a.cj0 + 1;
     a.k0 = k0;
+    a.nrow = nrow;
     if (a.nci > 0 && a.ncj > 0) {
       a.nbi = (a.nci + AB_TI - 1) / AB_TI;
       const int nbj = (a.ncj + AB_TJ - 1) / AB_TJ;
@@ -369,7 +400,7 @@ int launch_a2b_ord4(hipStream_t s, const fv3_domain* dom, const Met<T>& m, int k
       TimedLaunch timed("a2b_interp_kernel", s,
                         2.0 * sizeof(T) * (double)a.nci * a.ncj * nk * d.ntiles);
       hipLaunchKernelGGL((a2b_interp_kernel<T>), grid, dim3(AB_W, AB_NY), 0, s, d, m, a, qin,
-                         edges, qout);
+                         edges, xr, qout);
       FV3_HIP_CHECK_LAUNCH(who);
     }
   }
