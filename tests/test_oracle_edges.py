@@ -22,6 +22,7 @@ import pytest
 
 from fv3core_amd.workloads import plane_metric
 from oracle import csw as ocs
+from oracle.common import sh
 from oracle import dsw as ods
 from oracle.fvtp2d import Tile
 
@@ -520,3 +521,34 @@ def test_a2b_corner_extrapolation_is_exact_along_great_circles(corner):
     qout = np.zeros_like(q)
     ods.a2b_ord4(q, qout, g, t)
     np.testing.assert_allclose(qout[ci, cj], alpha, rtol=1e-9)
+
+
+def test_a2b_edge_forms_reduce_to_one_outer_and_one_mirrored_inner_form():
+    """The one-sided x volume means of a2b_ord4 (a2b_ord4.py:243-298) at the
+    four tile-edge interfaces are two forms, not four, in IEEE arithmetic:
+    qx_edge_east at a point is bit for bit qx_edge_west there (the same two
+    halves added in the other order); qx_edge_west2 builds on qx_edge_west of
+    the interface before it; and qx_edge_east2 is qx_edge_west2 of the data
+    mirrored about the interface.  The HIP volume means evaluate only the two
+    forms, one instruction stream for the edge lanes of a wave, on these
+    identities."""
+    rng = np.random.default_rng(4242)
+    n = 24
+    q = rng.uniform(-3.0, 3.0, (n, 5))
+    dx = rng.uniform(0.5, 2.0, (n, 5))
+    inner = slice(3, n - 3)
+    west, east = ods._qx_edge_west(q, dx), ods._qx_edge_east(q, dx)
+    np.testing.assert_array_equal(east[inner], west[inner])
+    # qx_edge_west2's left value is qx_edge_west one interface to the left
+    g_in = dx / sh(dx, -1)
+    left = sh(west, -1)
+    right = ods.B2 * (sh(q, -1) + sh(q, 2)) + ods.B1 * (q + sh(q, 1))
+    west2 = (3.0 * (g_in * sh(q, -1) + q) - (g_in * left + right)) / (2.0 + 2.0 * g_in)
+    np.testing.assert_array_equal(west2[inner], ods._qx_edge_west2(q, dx)[inner])
+    # interface p of the data reversed along i is interface n - p of the data
+    east2 = ods._qx_edge_east2(q, dx)
+    west2_rev = ods._qx_edge_west2(q[::-1].copy(), dx[::-1].copy())
+    assert np.isfinite(west[inner]).all() and np.isfinite(west2[inner]).all()
+    assert np.isfinite(east2[4:n - 3]).all()
+    for p in range(4, n - 3):
+        np.testing.assert_array_equal(east2[p], west2_rev[n - p])
